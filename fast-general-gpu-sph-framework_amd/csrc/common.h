@@ -1,0 +1,122 @@
+// common.h - device-side building blocks shared by the arrange and physics
+// kernels: scene geometry, the bit-exact micro hash, wave64 ballot helpers and
+// block-level scans.  Device-wide scans are reduce-then-scan (per-tile totals
+// written by one kernel, prefixes formed by the next): no block ever waits on
+// another, so no kernel can spin.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gsph.h"
+
+namespace gsph {
+
+constexpr int kWave = 64;
+constexpr int kSmsTaskParticles = 32;     // reference: src/core/sph_utils.cuh:35
+constexpr float kFloatSmall = 1e-12f;     // reference: src/core/cuda_math.cuh:14
+
+// Error bits collected in DevCounters::err and reported by the host.
+constexpr int kErrOutOfDomain = 1;
+constexpr int kErrCapacity = 4;
+
+// Slots of the small device counter block each context keeps.
+enum CounterSlot {
+    kCntK = 0,          // occupied cells
+    kCntT = 1,          // SMS tasks
+    kCntTra = 2,        // TRA particles
+    kCntSms = 3,        // SMS particles
+    kCntMiddle = 4,     // reference d_middle_value_ (-1 without SMS)
+    kCntErr = 5,
+    kCntSlots = 8,
+};
+
+// Geometry of one scene inside the (possibly batched) virtual grid.  In the
+// plain single-scene path there is exactly one, with cell_base = 0.
+struct SceneGeom {
+    int gx, gy, gz;
+    int cell_base;       // first global cell id of the scene
+    int p_begin;         // first sorted particle of the scene
+    int p_count;
+    float bound_min[3];
+    float bound_max[3];
+};
+
+// ParticlePos2CellPosM / CellPos2CellIdxM (reference src/core/sph_utils.cuh:53-105):
+// micro coordinate floor(p * (4 * (1/h))) per axis; coarse = micro >> 2;
+// id = coarse << 6 | (mx&3) << 4 | (mz&3) << 2 | (my&3).  Returns false when
+// the particle lies outside the scene grid (or is not finite).
+__device__ __forceinline__ bool micro_coords(float x, float y, float z,
+                                             float inv_cell_size,
+                                             const SceneGeom &g, int &mx,
+                                             int &my, int &mz) {
+    const float rat = 4.f * inv_cell_size;
+    const float fx = floorf(__fmul_rn(x, rat));
+    const float fy = floorf(__fmul_rn(y, rat));
+    const float fz = floorf(__fmul_rn(z, rat));
+    const bool ok = fx >= 0.f && fy >= 0.f && fz >= 0.f &&
+                    fx < (float)(g.gx * 4) && fy < (float)(g.gy * 4) &&
+                    fz < (float)(g.gz * 4);
+    mx = ok ? (int)fx : 0;
+    my = ok ? (int)fy : 0;
+    mz = ok ? (int)fz : 0;
+    return ok;
+}
+
+__device__ __forceinline__ uint32_t micro_key(int mx, int my, int mz,
+                                              const SceneGeom &g) {
+    const int cell = g.cell_base + (mx >> 2) + g.gx * ((my >> 2) + g.gy * (mz >> 2));
+    const int idi = (my & 3) + (((mz & 3) + ((mx & 3) << 2)) << 2);
+    return ((uint32_t)cell << 6) | (uint32_t)idi;
+}
+
+// ---- wave64 helpers --------------------------------------------------------
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const int lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Mask of the active lanes whose 9-bit value v equals this lane's.
+__device__ __forceinline__ uint64_t match9(uint32_t v) {
+    uint64_t peers = __ballot(1);
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        const bool bit = (v >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+// Block-wide exclusive scan of one uint32 per thread (blockDim.x = NT, a
+// multiple of 64).  Returns the exclusive prefix, writes the block total.
+template <int NT>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_tmp,
+                                                    uint32_t &total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // inclusive wave scan by shuffles
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < NT / 64; ++w) {
+            const uint32_t t = s_tmp[w];
+            s_tmp[w] = run;
+            run += t;
+        }
+        s_tmp[NT / 64] = run;
+    }
+    __syncthreads();
+    total = s_tmp[NT / 64];
+    const uint32_t r = s_tmp[wave] + x - v;
+    __syncthreads();
+    return r;
+}
+
+}  // namespace gsph

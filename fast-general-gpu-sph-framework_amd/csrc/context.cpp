@@ -1,0 +1,872 @@
+// context.cpp - the C ABI: contexts, device buffers, the step pipeline,
+// timing, graphs and read-back.
+//
+// One context holds one or more scenes (the plain path is a context with one
+// scene; gsph_batch_create builds a context whose scenes share every launch).
+// Device memory per particle: two copies of the state (A resident, B sorted),
+// acceleration, two radix key/value buffers, compact index; per cell: dense
+// begin/count plus sparse-written x-layer starts, req and TRA/SMS bases.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gsph.h"
+#include "host_util.h"
+#include "kernels.h"
+
+using namespace gsph;
+
+namespace {
+thread_local std::string g_thread_error;
+}
+
+struct KernelTimer {
+    std::vector<std::string> names;
+    std::vector<double> ms;
+    std::vector<int> launches;
+    struct Pending {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+struct gsph_ctx {
+    gsph_params params;
+    gsph_options opt;
+    gsph_constants c;
+    bool batch = false;
+    std::vector<SceneGeom> scenes;
+    int n = 0, numc = 0, passes = 0;
+    int sort_tiles = 0, cell_tiles = 0, nchunks = 0, split_tiles = 0, task_cap = 0;
+    int grid_blocks = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // device buffers
+    float4 *pos[2] = {nullptr, nullptr};
+    f3 *vel[2] = {nullptr, nullptr};
+    float4 *ev[2] = {nullptr, nullptr};
+    f3 *acc = nullptr;
+    uint32_t *keys[2] = {nullptr, nullptr};
+    int *vals[2] = {nullptr, nullptr};
+    uint32_t *tilecnt = nullptr, *digit_total = nullptr, *bcnt = nullptr, *split_sum = nullptr;
+    int *cells = nullptr, *cstart = nullptr, *chunk_k0 = nullptr;
+    int *begin = nullptr, *count = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
+    uint2 *xcum = nullptr;
+    int *compact = nullptr;
+    gsph_task *tasks = nullptr;
+    int *dc = nullptr;
+    SceneGeom *scenes_d = nullptr;
+    double *sum_partial = nullptr, *sum_out = nullptr;
+    std::vector<void *> allocs;
+    // state tracking
+    int cur = 0;              // buffer holding the current state (0 = A, 1 = B)
+    bool grid_valid = false;  // arrange outputs match the state in B
+    // graph replay of one full step
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    // timing
+    bool ktiming = false;
+    KernelTimer kt;
+    hipEvent_t stage_ev[5] = {};
+    int inject_after = -1;
+    std::string err;
+};
+
+namespace gsph {
+void set_error(gsph_ctx *ctx, const std::string &msg) {
+    if (ctx)
+        ctx->err = msg;
+    else
+        g_thread_error = msg;
+}
+}  // namespace gsph
+
+namespace {
+
+#define HIP_TRY(ctx, expr)                                                        \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return fail(ctx, GSPH_ERR_HIP,                                        \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <class T>
+int dalloc(gsph_ctx *ctx, T **p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(ctx, GSPH_ERR_HIP, "hipMalloc(" + std::to_string(bytes) +
+                                           " B): " + hipGetErrorString(e));
+    ctx->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return GSPH_OK;
+}
+
+int check_launch(gsph_ctx *ctx, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(ctx, GSPH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return GSPH_OK;
+}
+
+int consume_injection(gsph_ctx *ctx) {
+    if (ctx->inject_after < 0) return GSPH_OK;
+    if (ctx->inject_after == 0) {
+        ctx->inject_after = -1;
+        return fail(ctx, GSPH_ERR_INJECTED, "injected fault");
+    }
+    --ctx->inject_after;
+    return GSPH_OK;
+}
+
+// Kernel timing: brackets a launch with events when enabled.
+struct Timed {
+    gsph_ctx *ctx;
+    int id = -1;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(gsph_ctx *c, const char *name) : ctx(c) {
+        if (!ctx->ktiming) return;
+        KernelTimer &kt = ctx->kt;
+        for (size_t i = 0; i < kt.names.size(); ++i)
+            if (kt.names[i] == name) id = (int)i;
+        if (id < 0) {
+            id = (int)kt.names.size();
+            kt.names.push_back(name);
+            kt.ms.push_back(0.0);
+            kt.launches.push_back(0);
+        }
+        for (hipEvent_t *e : {&a, &b}) {
+            if (!kt.pool.empty()) {
+                *e = kt.pool.back();
+                kt.pool.pop_back();
+            } else {
+                (void)hipEventCreate(e);
+            }
+        }
+        (void)hipEventRecord(a, ctx->stream);
+    }
+    ~Timed() {
+        if (id < 0) return;
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->kt.pending.push_back({id, a, b});
+    }
+};
+
+void drain_timer(gsph_ctx *ctx) {
+    KernelTimer &kt = ctx->kt;
+    if (kt.pending.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &p : kt.pending) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        kt.ms[p.id] += ms;
+        kt.launches[p.id] += 1;
+        kt.pool.push_back(p.a);
+        kt.pool.push_back(p.b);
+    }
+    kt.pending.clear();
+}
+
+ArrangeArgs arrange_args(gsph_ctx *ctx) {
+    ArrangeArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = ctx->n;
+    a.numc = ctx->numc;
+    a.nscenes = (int)ctx->scenes.size();
+    a.scenes = ctx->scenes_d;
+    a.inv_cell_size = ctx->c.inv_cell_size;
+    a.split_mode = ctx->opt.split_mode;
+    a.sms_min_self = ctx->opt.sms_min_self;
+    a.sms_min_total = ctx->opt.sms_min_total;
+    a.pos_a = ctx->pos[0];
+    a.vel_a = ctx->vel[0];
+    a.ev_a = ctx->ev[0];
+    a.pos_b = ctx->pos[1];
+    a.vel_b = ctx->vel[1];
+    a.ev_b = ctx->ev[1];
+    for (int k = 0; k < 2; ++k) {
+        a.keys[k] = ctx->keys[k];
+        a.vals[k] = ctx->vals[k];
+    }
+    a.passes = ctx->passes;
+    a.sort_tiles = ctx->sort_tiles;
+    a.tilecnt = ctx->tilecnt;
+    a.digit_total = ctx->digit_total;
+    a.hash = ctx->keys[ctx->passes & 1];
+    a.perm = ctx->vals[ctx->passes & 1];
+    a.cell_tiles = ctx->cell_tiles;
+    a.bcnt = ctx->bcnt;
+    a.cells = ctx->cells;
+    a.cstart = ctx->cstart;
+    a.nchunks = ctx->nchunks;
+    a.chunk_k0 = ctx->chunk_k0;
+    a.begin = ctx->begin;
+    a.count = ctx->count;
+    a.xcum = ctx->xcum;
+    a.req = ctx->req;
+    a.tra_base = ctx->tra_base;
+    a.sms_base = ctx->sms_base;
+    a.split_tiles = ctx->split_tiles;
+    a.split_sum = ctx->split_sum;
+    a.compact = ctx->compact;
+    a.tasks = ctx->tasks;
+    a.task_cap = ctx->task_cap;
+    a.dc = ctx->dc;
+    return a;
+}
+
+PhysicsArgs physics_args(gsph_ctx *ctx) {
+    PhysicsArgs p;
+    std::memset(&p, 0, sizeof(p));
+    p.n = ctx->n;
+    p.nscenes = (int)ctx->scenes.size();
+    p.scenes = ctx->scenes_d;
+    p.c = ctx->c;
+    p.pos = ctx->pos[1];
+    p.vel = ctx->vel[1];
+    p.ev = ctx->ev[1];
+    p.acc = ctx->acc;
+    p.pos_out = ctx->pos[0];
+    p.vel_out = ctx->vel[0];
+    p.ev_out = ctx->ev[0];
+    p.hash = ctx->keys[ctx->passes & 1];
+    p.begin = ctx->begin;
+    p.xcum = ctx->xcum;
+    p.compact = ctx->compact;
+    p.tasks = ctx->tasks;
+    p.dc = ctx->dc;
+    p.grid_blocks = ctx->grid_blocks;
+    p.row_prune = ctx->opt.row_prune;
+    return p;
+}
+
+int do_arrange(gsph_ctx *ctx) {
+    const ArrangeArgs a = arrange_args(ctx);
+    hipStream_t s = ctx->stream;
+    if (ctx->n == 0) {
+        { Timed t(ctx, "empty_grid"); launch_empty_grid(a, s); }
+        { Timed t(ctx, "fill"); launch_fill(a, s); }
+        { Timed t(ctx, "compact"); launch_compact(a, s); }
+    } else {
+        { Timed t(ctx, "hash"); launch_hash(a, ctx->batch, s); }
+        { Timed t(ctx, "rowscan"); launch_rowscan(a, s); }
+        { Timed t(ctx, "downsweep"); launch_downsweep(a, 0, s); }
+        for (int p = 1; p < ctx->passes; ++p) {
+            { Timed t(ctx, "upsweep"); launch_upsweep(a, p, s); }
+            { Timed t(ctx, "rowscan"); launch_rowscan(a, s); }
+            { Timed t(ctx, "downsweep"); launch_downsweep(a, p, s); }
+        }
+        { Timed t(ctx, "gather"); launch_gather(a, s); }
+        { Timed t(ctx, "cells"); launch_cells(a, s); }
+        { Timed t(ctx, "fill"); launch_fill(a, s); }
+        { Timed t(ctx, "split_count"); launch_split_count(a, ctx->batch, s); }
+        { Timed t(ctx, "split_apply"); launch_split_apply(a, ctx->batch, s); }
+        { Timed t(ctx, "compact"); launch_compact(a, s); }
+    }
+    int rc = check_launch(ctx, "arrange");
+    if (rc) return rc;
+    ctx->cur = 1;
+    ctx->grid_valid = true;
+    return GSPH_OK;
+}
+
+int do_density(gsph_ctx *ctx) {
+    if (ctx->n == 0) return GSPH_OK;
+    const PhysicsArgs p = physics_args(ctx);
+    { Timed t(ctx, "density"); launch_density(p, ctx->batch, ctx->stream); }
+    return check_launch(ctx, "density");
+}
+
+int do_force(gsph_ctx *ctx) {
+    if (ctx->n == 0) return GSPH_OK;
+    const PhysicsArgs p = physics_args(ctx);
+    { Timed t(ctx, "force"); launch_force(p, ctx->batch, ctx->stream); }
+    return check_launch(ctx, "force");
+}
+
+int do_integrate(gsph_ctx *ctx) {
+    if (ctx->n > 0) {
+        const PhysicsArgs p = physics_args(ctx);
+        { Timed t(ctx, "integrate"); launch_integrate(p, ctx->batch, ctx->stream); }
+        int rc = check_launch(ctx, "integrate");
+        if (rc) return rc;
+    }
+    ctx->cur = 0;
+    ctx->grid_valid = false;
+    return GSPH_OK;
+}
+
+int do_step(gsph_ctx *ctx) {
+    int rc = do_arrange(ctx);
+    if (!rc) rc = do_density(ctx);
+    if (!rc) rc = do_force(ctx);
+    if (!rc) rc = do_integrate(ctx);
+    return rc;
+}
+
+int collect_errors(gsph_ctx *ctx) {
+    int flags = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&flags, ctx->dc + kCntErr, sizeof(int),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (flags) {
+        const int zero = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->dc + kCntErr, &zero, sizeof(int),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (flags & kErrOutOfDomain)
+            return fail(ctx, GSPH_ERR_OUT_OF_DOMAIN,
+                        "a particle left the grid (or is not finite)");
+        if (flags & kErrCapacity)
+            return fail(ctx, GSPH_ERR_CAPACITY,
+                        "task capacity exceeded or a cell holds more than 65535 particles");
+    }
+    return GSPH_OK;
+}
+
+int bits_for(long long maxval) {
+    int b = 1;
+    while (b < 40 && (1LL << b) <= maxval) ++b;
+    return b;
+}
+
+int build_context(const gsph_params *p, const gsph_options *opt,
+                  const std::vector<SceneGeom> &scenes_in, int n, const float *pos4,
+                  const float *vel3, bool batch, gsph_ctx **out) {
+    if (!p || !out) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    if (n < 0) return fail(nullptr, GSPH_ERR_INVALID_ARG, "negative particle count");
+    if (n > 0 && !pos4) return fail(nullptr, GSPH_ERR_INVALID_ARG, "pos4 is null");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, GSPH_ERR_NO_DEVICE, "There is no HIP device.");
+    gsph_ctx *ctx = new gsph_ctx();
+    ctx->params = *p;
+    if (opt)
+        ctx->opt = *opt;
+    else
+        gsph_options_default(&ctx->opt);
+    int rc = gsph_derive_constants(p, &ctx->c);
+    if (rc) {
+        delete ctx;
+        return rc;
+    }
+    ctx->batch = batch;
+    ctx->scenes = scenes_in;
+    ctx->n = n;
+    long long numc = 0;
+    for (auto &g : ctx->scenes) {
+        if (g.gx <= 0 || g.gy <= 0 || g.gz <= 0 || g.gx > 16383 || g.gy > 16383 || g.gz > 16383) {
+            delete ctx;
+            return fail(nullptr, GSPH_ERR_INVALID_ARG, "grid dimensions must be in [1, 16383]");
+        }
+        g.cell_base = (int)numc;
+        numc += (long long)g.gx * g.gy * g.gz;
+        if (numc > (1LL << 26)) {
+            delete ctx;
+            return fail(nullptr, GSPH_ERR_CAPACITY,
+                        "more than 2^26 grid cells (micro ids must fit 32 bits)");
+        }
+    }
+    ctx->numc = (int)numc;
+    ctx->passes = (bits_for(numc * 64 - 1) + 7) / 8;
+    ctx->sort_tiles = std::max(1, (n + kSortTile - 1) / kSortTile);
+    ctx->cell_tiles = std::max(1, (n + kCellTile - 1) / kCellTile);
+    ctx->nchunks = std::max(1, (ctx->numc + kFillChunk - 1) / kFillChunk);
+    ctx->split_tiles = std::max(1, (n + kSplitTile - 1) / kSplitTile);
+    ctx->task_cap = (n + 31) / 32 + std::min(ctx->numc, n) + 2;
+
+    hipDeviceProp_t prop;
+    (void)hipGetDevice(&ctx->device);
+    int cus = 256;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;
+    int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 8;
+    ctx->grid_blocks = std::max(8, (cus * bpc + 7) / 8 * 8);
+
+#define ALLOC(ptr, cnt)                         \
+    do {                                        \
+        rc = dalloc(ctx, &(ptr), (size_t)(cnt)); \
+        if (rc) goto failed;                    \
+    } while (0)
+    {
+        const size_t N = (size_t)n, C = (size_t)ctx->numc;
+        for (int k = 0; k < 2; ++k) {
+            ALLOC(ctx->pos[k], N);
+            ALLOC(ctx->vel[k], N);
+            ALLOC(ctx->ev[k], N);
+            ALLOC(ctx->keys[k], N);
+            ALLOC(ctx->vals[k], N);
+        }
+        ALLOC(ctx->acc, N);
+        ALLOC(ctx->tilecnt, 256 * (size_t)ctx->sort_tiles);
+        ALLOC(ctx->digit_total, 256);
+        ALLOC(ctx->bcnt, ctx->cell_tiles);
+        ALLOC(ctx->split_sum, 3 * (size_t)ctx->split_tiles);
+        ALLOC(ctx->cells, N + 1);
+        ALLOC(ctx->cstart, N + 1);
+        ALLOC(ctx->chunk_k0, ctx->nchunks + 1);
+        ALLOC(ctx->begin, C + 1);
+        ALLOC(ctx->count, C);
+        ALLOC(ctx->req, C);
+        ALLOC(ctx->tra_base, C);
+        ALLOC(ctx->sms_base, C);
+        ALLOC(ctx->xcum, C);
+        ALLOC(ctx->compact, N);
+        ALLOC(ctx->tasks, ctx->task_cap);
+        ALLOC(ctx->dc, kCntSlots);
+        ALLOC(ctx->scenes_d, ctx->scenes.size());
+        ALLOC(ctx->sum_partial, 5 * 256);
+        ALLOC(ctx->sum_out, 5);
+    }
+#undef ALLOC
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        rc = fail(nullptr, GSPH_ERR_HIP, "hipStreamCreate failed");
+        goto failed;
+    }
+    for (auto &e : ctx->stage_ev) (void)hipEventCreate(&e);
+    {
+        hipError_t e = hipMemcpyAsync(ctx->scenes_d, ctx->scenes.data(),
+                                      sizeof(SceneGeom) * ctx->scenes.size(),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(ctx->dc, 0, sizeof(int) * kCntSlots, ctx->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(ctx->tasks, 0, sizeof(gsph_task) * ctx->task_cap, ctx->stream);
+        if (e == hipSuccess && n > 0)
+            e = hipMemcpyAsync(ctx->pos[0], pos4, sizeof(float4) * (size_t)n,
+                               hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess && n > 0) {
+            if (vel3)
+                e = hipMemcpyAsync(ctx->vel[0], vel3, sizeof(f3) * (size_t)n,
+                                   hipMemcpyHostToDevice, ctx->stream);
+            else
+                e = hipMemsetAsync(ctx->vel[0], 0, sizeof(f3) * (size_t)n, ctx->stream);
+        }
+        if (e != hipSuccess) {
+            rc = fail(nullptr, GSPH_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+            goto failed;
+        }
+        launch_init_state(ctx->ev[0], ctx->acc, n, ctx->stream);
+        e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) {
+            rc = fail(nullptr, GSPH_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
+            goto failed;
+        }
+    }
+    ctx->cur = 0;
+    *out = ctx;
+    return GSPH_OK;
+failed:
+    gsph_destroy(ctx);
+    return rc;
+}
+
+SceneGeom geom_for(const gsph_constants &c, const float world[3], int p_begin, int p_count) {
+    SceneGeom g;
+    std::memset(&g, 0, sizeof(g));
+    for (int a = 0; a < 3; ++a) {
+        const float cells = world[a] / c.cell_size;
+        const int gs = (int)std::ceil(cells);
+        (&g.gx)[a] = gs;
+        g.bound_min[a] = c.kernel_h;
+        g.bound_max[a] = world[a] - c.kernel_h;
+    }
+    g.p_begin = p_begin;
+    g.p_count = p_count;
+    return g;
+}
+
+int capture_graph(gsph_ctx *ctx) {
+    if (ctx->graph_exec) {
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
+    if (ctx->graph) {
+        (void)hipGraphDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const bool kt = ctx->ktiming;
+    ctx->ktiming = false;
+    int rc = do_step(ctx);
+    ctx->ktiming = kt;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, GSPH_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    ctx->graph = g;
+    HIP_TRY(ctx, hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+    return GSPH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *gsph_last_error(const gsph_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_thread_error.c_str();
+}
+
+int gsph_device_count(int *count) {
+    if (!count) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return GSPH_OK;
+}
+
+int gsph_create(const gsph_params *p, const gsph_options *opt, int n, const float *pos4,
+                const float *vel3, gsph_ctx **out) {
+    if (!p) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null params");
+    gsph_constants c;
+    int rc = gsph_derive_constants(p, &c);
+    if (rc) return rc;
+    std::vector<SceneGeom> sc(1, geom_for(c, p->world_size, 0, n));
+    return build_context(p, opt, sc, n, pos4, vel3, false, out);
+}
+
+int gsph_batch_create(const gsph_params *p, const gsph_options *opt, int nscenes,
+                      const float *world3, const int *counts, const float *pos4,
+                      const float *vel3, gsph_ctx **out) {
+    if (!p || nscenes <= 0 || !world3 || !counts)
+        return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad batch arguments");
+    gsph_constants c;
+    int rc = gsph_derive_constants(p, &c);
+    if (rc) return rc;
+    std::vector<SceneGeom> sc;
+    long long total = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        if (counts[s] < 0) return fail(nullptr, GSPH_ERR_INVALID_ARG, "negative scene count");
+        sc.push_back(geom_for(c, world3 + 3 * s, (int)total, counts[s]));
+        total += counts[s];
+        if (total > 0x7fffffffLL) return fail(nullptr, GSPH_ERR_CAPACITY, "too many particles");
+    }
+    return build_context(p, opt, sc, (int)total, pos4, vel3, true, out);
+}
+
+int gsph_scene_count(const gsph_ctx *ctx, int *nscenes) {
+    if (!ctx || !nscenes) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    *nscenes = (int)ctx->scenes.size();
+    return GSPH_OK;
+}
+
+int gsph_scene_range(const gsph_ctx *ctx, int s, int *first, int *count) {
+    if (!ctx || !first || !count || s < 0 || s >= (int)ctx->scenes.size())
+        return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad scene index");
+    *first = ctx->scenes[s].p_begin;
+    *count = ctx->scenes[s].p_count;
+    return GSPH_OK;
+}
+
+int gsph_destroy(gsph_ctx *ctx) {
+    if (!ctx) return GSPH_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    for (auto &e : ctx->stage_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &p : ctx->kt.pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->kt.pool) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GSPH_OK;
+}
+
+int gsph_num_particles(const gsph_ctx *ctx, int *n) {
+    if (!ctx || !n) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    *n = ctx->n;
+    return GSPH_OK;
+}
+
+int gsph_get_constants(const gsph_ctx *ctx, gsph_constants *out) {
+    if (!ctx || !out) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    *out = ctx->c;
+    return GSPH_OK;
+}
+
+int gsph_set_state(gsph_ctx *ctx, const float *pos4, const float *vel3, const float *ev4) {
+    if (!ctx || (ctx->n > 0 && !pos4)) return fail(ctx, GSPH_ERR_INVALID_ARG, "null argument");
+    const size_t n = (size_t)ctx->n;
+    if (n) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->pos[0], pos4, sizeof(float4) * n,
+                                    hipMemcpyHostToDevice, ctx->stream));
+        if (vel3)
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->vel[0], vel3, sizeof(f3) * n,
+                                        hipMemcpyHostToDevice, ctx->stream));
+        else
+            HIP_TRY(ctx, hipMemsetAsync(ctx->vel[0], 0, sizeof(f3) * n, ctx->stream));
+        if (ev4)
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->ev[0], ev4, sizeof(float4) * n,
+                                        hipMemcpyHostToDevice, ctx->stream));
+        else
+            HIP_TRY(ctx, hipMemsetAsync(ctx->ev[0], 0, sizeof(float4) * n, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->cur = 0;
+    ctx->grid_valid = false;
+    return GSPH_OK;
+}
+
+int gsph_arrange(gsph_ctx *ctx) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    int rc = consume_injection(ctx);
+    if (rc) return rc;
+    if (ctx->cur != 0)
+        return fail(ctx, GSPH_ERR_INVALID_ARG,
+                    "arrange expects the integrated state (call integrate first)");
+    return do_arrange(ctx);
+}
+
+int gsph_density(gsph_ctx *ctx) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    int rc = consume_injection(ctx);
+    if (rc) return rc;
+    if (!ctx->grid_valid) return fail(ctx, GSPH_ERR_INVALID_ARG, "density needs arrange first");
+    return do_density(ctx);
+}
+
+int gsph_force(gsph_ctx *ctx) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    int rc = consume_injection(ctx);
+    if (rc) return rc;
+    if (!ctx->grid_valid) return fail(ctx, GSPH_ERR_INVALID_ARG, "force needs arrange first");
+    return do_force(ctx);
+}
+
+int gsph_integrate(gsph_ctx *ctx) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    int rc = consume_injection(ctx);
+    if (rc) return rc;
+    if (!ctx->grid_valid) return fail(ctx, GSPH_ERR_INVALID_ARG, "integrate needs arrange first");
+    return do_integrate(ctx);
+}
+
+int gsph_step(gsph_ctx *ctx, int nsteps, gsph_stage_ms *per_step_ms) {
+    if (!ctx || nsteps < 0) return fail(ctx, GSPH_ERR_INVALID_ARG, "bad arguments");
+    int rc = consume_injection(ctx);
+    if (rc) return rc;
+    if (ctx->cur != 0)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "step expects the integrated state");
+    for (int s = 0; s < nsteps; ++s) {
+        if (per_step_ms) {
+            hipEvent_t *e = ctx->stage_ev;
+            (void)hipEventRecord(e[0], ctx->stream);
+            rc = do_arrange(ctx);
+            (void)hipEventRecord(e[1], ctx->stream);
+            if (!rc) rc = do_density(ctx);
+            (void)hipEventRecord(e[2], ctx->stream);
+            if (!rc) rc = do_force(ctx);
+            (void)hipEventRecord(e[3], ctx->stream);
+            if (!rc) rc = do_integrate(ctx);
+            (void)hipEventRecord(e[4], ctx->stream);
+            if (rc) return rc;
+            HIP_TRY(ctx, hipEventSynchronize(e[4]));
+            gsph_stage_ms &o = per_step_ms[s];
+            (void)hipEventElapsedTime(&o.arrange, e[0], e[1]);
+            (void)hipEventElapsedTime(&o.density, e[1], e[2]);
+            (void)hipEventElapsedTime(&o.force, e[2], e[3]);
+            (void)hipEventElapsedTime(&o.integrate, e[3], e[4]);
+            (void)hipEventElapsedTime(&o.total, e[0], e[4]);
+        } else if (ctx->use_graph && !ctx->ktiming) {
+            if (!ctx->graph_exec) {
+                rc = capture_graph(ctx);
+                if (rc) return rc;
+            }
+            HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+            ctx->cur = 0;
+            ctx->grid_valid = false;
+        } else {
+            rc = do_step(ctx);
+            if (rc) return rc;
+        }
+    }
+    return GSPH_OK;
+}
+
+int gsph_enable_graph(gsph_ctx *ctx, int enable) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    ctx->use_graph = enable != 0;
+    return GSPH_OK;
+}
+
+int gsph_sync(gsph_ctx *ctx) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return collect_errors(ctx);
+}
+
+int gsph_read_state(gsph_ctx *ctx, float *pos4, float *vel3, float *ev4, float *acc3) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    const size_t n = (size_t)ctx->n;
+    const int b = ctx->cur;
+    if (n) {
+        if (pos4)
+            HIP_TRY(ctx, hipMemcpyAsync(pos4, ctx->pos[b], sizeof(float4) * n,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+        if (vel3)
+            HIP_TRY(ctx, hipMemcpyAsync(vel3, ctx->vel[b], sizeof(f3) * n,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+        if (ev4)
+            HIP_TRY(ctx, hipMemcpyAsync(ev4, ctx->ev[b], sizeof(float4) * n,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+        if (acc3)
+            HIP_TRY(ctx, hipMemcpyAsync(acc3, ctx->acc, sizeof(f3) * n,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return collect_errors(ctx);
+}
+
+int gsph_grid_info(gsph_ctx *ctx, int *cells, int *tasks, int *middle_raw, int *tra) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    int dc[kCntSlots];
+    HIP_TRY(ctx, hipMemcpyAsync(dc, ctx->dc, sizeof(dc), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (cells) *cells = ctx->numc;
+    if (tasks) *tasks = dc[kCntT];
+    if (middle_raw) *middle_raw = dc[kCntMiddle];
+    if (tra) *tra = dc[kCntTra];
+    return collect_errors(ctx);
+}
+
+int gsph_read_grid(gsph_ctx *ctx, int *hash, int *perm, int *begin, int *count, int *req,
+                   int *compact, gsph_task *tasks) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    if (!ctx->grid_valid) return fail(ctx, GSPH_ERR_INVALID_ARG, "no arranged grid to read");
+    const size_t n = (size_t)ctx->n, C = (size_t)ctx->numc;
+    hipStream_t s = ctx->stream;
+    int dc[kCntSlots];
+    HIP_TRY(ctx, hipMemcpyAsync(dc, ctx->dc, sizeof(dc), hipMemcpyDeviceToHost, s));
+    if (n && hash)
+        HIP_TRY(ctx, hipMemcpyAsync(hash, ctx->keys[ctx->passes & 1], 4 * n, hipMemcpyDeviceToHost, s));
+    if (n && perm)
+        HIP_TRY(ctx, hipMemcpyAsync(perm, ctx->vals[ctx->passes & 1], 4 * n, hipMemcpyDeviceToHost, s));
+    if (begin) HIP_TRY(ctx, hipMemcpyAsync(begin, ctx->begin, 4 * C, hipMemcpyDeviceToHost, s));
+    std::vector<int> cnt_tmp;
+    int *cnt = count;
+    if (req && !cnt) {
+        cnt_tmp.resize(C);
+        cnt = cnt_tmp.data();
+    }
+    if (cnt) HIP_TRY(ctx, hipMemcpyAsync(cnt, ctx->count, 4 * C, hipMemcpyDeviceToHost, s));
+    if (req) HIP_TRY(ctx, hipMemcpyAsync(req, ctx->req, 4 * C, hipMemcpyDeviceToHost, s));
+    if (n && compact)
+        HIP_TRY(ctx, hipMemcpyAsync(compact, ctx->compact, 4 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (req)
+        for (size_t c = 0; c < C; ++c)
+            if (cnt[c] == 0) req[c] = 0;  // the sparse table is only valid in occupied cells
+    if (tasks) {
+        const int T = dc[kCntT];
+        const int nd = T + (T & 1);
+        if (nd > 0)
+            HIP_TRY(ctx, hipMemcpy(tasks, ctx->tasks, sizeof(gsph_task) * nd, hipMemcpyDeviceToHost));
+    }
+    return collect_errors(ctx);
+}
+
+int gsph_checksum_range(gsph_ctx *ctx, int first, int count, double out[5]) {
+    if (!ctx || !out || first < 0 || count < 0 || first + count > ctx->n)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "bad checksum range");
+    launch_checksum(ctx->pos[ctx->cur] + first, count, ctx->sum_partial, 256, ctx->sum_out,
+                    ctx->stream);
+    int rc = check_launch(ctx, "checksum");
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(out, ctx->sum_out, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return collect_errors(ctx);
+}
+
+int gsph_checksum(gsph_ctx *ctx, double out[5]) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    return gsph_checksum_range(ctx, 0, ctx->n, out);
+}
+
+int gsph_get_pair_stats(gsph_ctx *ctx, gsph_pair_stats *out) {
+    if (!ctx || !out) return fail(ctx, GSPH_ERR_INVALID_ARG, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    int dc[kCntSlots];
+    HIP_TRY(ctx, hipMemcpyAsync(dc, ctx->dc, sizeof(dc), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const int T = dc[kCntT];
+    if (T <= 0) return GSPH_OK;
+    std::vector<gsph_task> tk(T + (T & 1));
+    HIP_TRY(ctx, hipMemcpy(tk.data(), ctx->tasks, sizeof(gsph_task) * tk.size(),
+                           hipMemcpyDeviceToHost));
+    out->total_pairs = (T + 1) / 2;
+    for (int t = 0; t < T; t += 2) {
+        const gsph_task &f = tk[t], &s = tk[t + 1];
+        if (f.is_same) {
+            ++out->is_same_pairs;
+            if (f.cell_particle_count - s.p_offset >= kSmsTaskParticles)
+                ++out->full_is_same_pairs;
+            else
+                ++out->partial_is_same_pairs;
+            if (f.xxi == s.xxx)
+                ++out->compact_is_same_pairs;
+            else
+                ++out->widened_is_same_pairs;
+        }
+        if (f.cellid == s.cellid) ++out->same_cell_pairs;
+    }
+    return GSPH_OK;
+}
+
+int gsph_kernel_timing(gsph_ctx *ctx, int enable) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    drain_timer(ctx);
+    ctx->ktiming = enable != 0;
+    if (enable) {
+        std::fill(ctx->kt.ms.begin(), ctx->kt.ms.end(), 0.0);
+        std::fill(ctx->kt.launches.begin(), ctx->kt.launches.end(), 0);
+    }
+    return GSPH_OK;
+}
+
+int gsph_kernel_times(gsph_ctx *ctx, char *names, int names_cap, float *ms, int *launches,
+                      int cap, int *count) {
+    if (!ctx || !count) return fail(ctx, GSPH_ERR_INVALID_ARG, "null argument");
+    drain_timer(ctx);
+    const KernelTimer &kt = ctx->kt;
+    *count = (int)kt.names.size();
+    std::string joined;
+    for (size_t i = 0; i < kt.names.size(); ++i) {
+        if (i) joined += ';';
+        joined += kt.names[i];
+        if ((int)i < cap) {
+            if (ms) ms[i] = (float)kt.ms[i];
+            if (launches) launches[i] = kt.launches[i];
+        }
+    }
+    if (names && names_cap > 0) {
+        std::strncpy(names, joined.c_str(), (size_t)names_cap - 1);
+        names[names_cap - 1] = 0;
+    }
+    return GSPH_OK;
+}
+
+int gsph_inject_fault(gsph_ctx *ctx, int after_calls) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    ctx->inject_after = after_calls;
+    return GSPH_OK;
+}
+
+}  // extern "C"

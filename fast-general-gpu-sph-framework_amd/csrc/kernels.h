@@ -1,0 +1,113 @@
+// kernels.h - argument blocks and host launch wrappers of the HIP kernels.
+// The context layer (context.cpp) fills these structs; arrange.hip and
+// physics.hip define the kernels and the launchers declared here.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+#include "gsph.h"
+
+namespace gsph {
+
+struct f3 {
+    float x, y, z;
+};
+
+constexpr int kSortTile = 4096;     // keys per radix tile (256 threads x 16)
+constexpr int kCellTile = 4096;     // sorted particles per cell-table tile
+constexpr int kFillChunk = 4096;    // cells per fill block
+constexpr int kSplitTile = 1024;    // occupied cells per split tile
+
+// Everything the arrange kernels touch.  Device pointers.
+struct ArrangeArgs {
+    int n;                 // particles
+    int numc;              // cells of the (virtual) grid
+    int nscenes;
+    const SceneGeom *scenes;
+    float inv_cell_size;
+    int split_mode;        // 0 reference heuristic, 1 tuned thresholds
+    int sms_min_self, sms_min_total;
+    // state: A = resident (input of the sort), B = sorted working copy
+    const float4 *pos_a;
+    const f3 *vel_a;
+    const float4 *ev_a;
+    float4 *pos_b;
+    f3 *vel_b;
+    float4 *ev_b;
+    // radix sort
+    uint32_t *keys[2];
+    int *vals[2];
+    int passes;
+    int sort_tiles;
+    uint32_t *tilecnt;     // [256][sort_tiles]
+    uint32_t *digit_total; // [256]
+    // results
+    uint32_t *hash;        // sorted keys (== keys[passes & 1])
+    int *perm;             // sorted slot -> previous slot (== vals[passes & 1])
+    int cell_tiles;
+    uint32_t *bcnt;        // [cell_tiles] boundary counts
+    int *cells;            // [n+1] occupied cell ids
+    int *cstart;           // [n+1] first sorted particle of each occupied cell
+    int nchunks;
+    int *chunk_k0;         // [nchunks] first occupied cell index of each chunk
+    int *begin;            // [numc+1]
+    int *count;            // [numc]
+    uint2 *xcum;           // [numc] x-layer starts (valid where count > 0)
+    int *req;              // [numc] valid where count > 0
+    int *tra_base;         // [numc] valid where count > 0
+    int *sms_base;         // [numc] valid where count > 0
+    int split_tiles;
+    uint32_t *split_sum;   // [3][split_tiles]
+    int *compact;          // [n]
+    gsph_task *tasks;      // [task_cap]
+    int task_cap;
+    int *dc;               // counters (CounterSlot)
+};
+
+// Density / force / integrate.
+struct PhysicsArgs {
+    int n;
+    int nscenes;
+    const SceneGeom *scenes;
+    gsph_constants c;
+    float4 *pos;           // sorted state (B)
+    f3 *vel;
+    float4 *ev;
+    f3 *acc;
+    float4 *pos_out;       // integrate destination (A)
+    f3 *vel_out;
+    float4 *ev_out;
+    const uint32_t *hash;
+    const int *begin;
+    const uint2 *xcum;
+    const int *compact;
+    const gsph_task *tasks;
+    const int *dc;
+    int grid_blocks;       // persistent grid of density / force
+    int row_prune;
+};
+
+// arrange.hip
+void launch_hash(const ArrangeArgs &a, bool batch, hipStream_t s);
+void launch_upsweep(const ArrangeArgs &a, int pass, hipStream_t s);
+void launch_rowscan(const ArrangeArgs &a, hipStream_t s);
+void launch_downsweep(const ArrangeArgs &a, int pass, hipStream_t s);
+void launch_gather(const ArrangeArgs &a, hipStream_t s);
+void launch_cells(const ArrangeArgs &a, hipStream_t s);
+void launch_fill(const ArrangeArgs &a, hipStream_t s);
+void launch_split_count(const ArrangeArgs &a, bool batch, hipStream_t s);
+void launch_split_apply(const ArrangeArgs &a, bool batch, hipStream_t s);
+void launch_compact(const ArrangeArgs &a, hipStream_t s);
+void launch_empty_grid(const ArrangeArgs &a, hipStream_t s);
+
+// physics.hip
+void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s);
+void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s);
+void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s);
+void launch_checksum(const float4 *pos, int n, double *partial, int nblocks,
+                     double *out5, hipStream_t s);
+void launch_init_state(float4 *ev, f3 *acc, int n, hipStream_t s);
+
+}  // namespace gsph

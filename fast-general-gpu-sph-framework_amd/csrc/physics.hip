@@ -1,0 +1,523 @@
+// physics.hip - density, force and integration kernels of one SPH step on
+// gfx950, plus the FP64 state checksum.
+//
+// Work items of the density and force kernels (one 64-lane wave each):
+//   TRA tile  - 64 consecutive entries of compact[0, tra): one particle per
+//               lane, each lane walks its own 9 neighbour rows (the reference's
+//               TRA path, density_kernels.cu:323-367, force_kernels.cu:749-818);
+//   SMS pair  - tasks 2q and 2q+1: each half-wave owns one 32-particle task and
+//               streams its rows through LDS 32 candidates at a time, every lane
+//               testing all staged candidates against its particle (the
+//               reference's register SMS path, density_kernels.cu:370-545).
+// A row (dy, dz) is the sorted range from the left cell's micro-x layer L to
+// the right cell's layer R (findTraNeighborParticleRange,
+// kernel_common.cuh:69-112); cells x-1, x, x+1 of a row are contiguous in
+// sorted order.  With row_prune on, a row is skipped when its (y, z) slab lies
+// farther than h from the particle, and L/R shrink to the x reach left by the
+// slab distance: a superset of the interacting set, smaller than the
+// reference's.
+//
+// Grid: a persistent grid; item i goes to XCD (i * 8 / items) so that the
+// waves sharing one L2 work on one contiguous eighth of space, round-robin
+// among that XCD's waves.
+#include "kernels.h"
+
+namespace gsph {
+namespace {
+
+constexpr int kNT = 256;
+
+__device__ __forceinline__ float powf7(float b) {
+    return b * b * b * b * b * b * b;
+}
+
+template <bool kBatch>
+__device__ __forceinline__ int scene_of_particle(const SceneGeom *sc, int ns,
+                                                 int i) {
+    if (!kBatch) return 0;
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (sc[m].p_begin <= i)
+            lo = m;
+        else
+            hi = m - 1;
+    }
+    return lo;
+}
+
+// #particles of cell c whose micro x layer is < L (L in 0..4).
+__device__ __forceinline__ int layers_below(const int *begin, const uint2 *xcum,
+                                            int c, int L) {
+    if (L <= 0) return 0;
+    const int cnt = begin[c + 1] - begin[c];
+    if (L >= 4 || cnt == 0) return L >= 4 ? cnt : 0;
+    const uint2 x = xcum[c];
+    return L == 1 ? (int)(x.x & 0xffffu) : L == 2 ? (int)(x.x >> 16) : (int)x.y;
+}
+
+// Sorted range of row (ny, nz) for a particle of cell column cx whose
+// candidates span micro-x layers [gl, gr] (global micro-x indices).
+__device__ __forceinline__ void row_range(const SceneGeom &g, const int *begin,
+                                          const uint2 *xcum, int cx, int ny,
+                                          int nz, int gl, int gr, int &lo,
+                                          int &hi) {
+    const int rowc = g.cell_base + g.gx * (ny + g.gy * nz);
+    const int lcx = max(gl >> 2, 0);
+    const int rcx = min(gr >> 2, g.gx - 1);
+    const int lc = rowc + lcx, rc = rowc + rcx;
+    lo = begin[lc] + ((gl >> 2) == lcx ? layers_below(begin, xcum, lc, gl & 3) : 0);
+    hi = begin[rc] + ((gr >> 2) == rcx ? layers_below(begin, xcum, rc, (gr & 3) + 1)
+                                       : begin[rc + 1] - begin[rc]);
+    (void)cx;
+}
+
+struct RowPlan {
+    int ny[9], nz[9], gl[9], gr[9];
+    bool live[9];
+};
+
+// The 9 rows of a particle at (x, y, z) in micro cell (mx, my, mz).  Reference
+// rows: layers [mx - 4, mx + 4].  Pruned rows: the slab distance d of the row
+// in (y, z) gives the x reach sqrt(h^2 - d^2); layers outside [x - reach,
+// x + reach] are dropped (with a small relative margin, so that a pair the
+// FP32 test accepts is never pruned).
+__device__ __forceinline__ void plan_rows(const SceneGeom &g, const gsph_constants &c,
+                                          float x, float y, float z, int mx,
+                                          int my, int mz, bool prune, RowPlan &rp) {
+    const int cy = my >> 2, cz = mz >> 2;
+    const float h = c.cell_size;
+    const float rat = 4.f * c.inv_cell_size;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int dy = r % 3 - 1, dz = r / 3 - 1;
+        const int ny = cy + dy, nz = cz + dz;
+        rp.ny[r] = ny;
+        rp.nz[r] = nz;
+        rp.live[r] = ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz;
+        int gl = mx - 4, gr = mx + 4;
+        if (prune && rp.live[r]) {
+            const float ey = dy < 0 ? y - (float)cy * h : dy > 0 ? (float)(cy + 1) * h - y : 0.f;
+            const float ez = dz < 0 ? z - (float)cz * h : dz > 0 ? (float)(cz + 1) * h - z : 0.f;
+            const float d2 = fmaxf(ey, 0.f) * fmaxf(ey, 0.f) + fmaxf(ez, 0.f) * fmaxf(ez, 0.f);
+            // margins: 1e-3 relative on h^2 and the reach plus 1e-4 h absolute,
+            // far above the FP32 rounding of coordinates up to 2^14 h
+            const float lim = c.kernel_2 * 1.001f;
+            if (d2 > lim) {
+                rp.live[r] = false;
+            } else {
+                const float reach = sqrtf(fmaxf(lim - d2, 0.f)) * 1.001f + 1e-4f * h;
+                gl = max(gl, (int)floorf((x - reach) * rat));
+                gr = min(gr, (int)floorf((x + reach) * rat));
+            }
+        }
+        rp.gl[r] = max(gl, 0);
+        rp.gr[r] = min(gr, g.gx * 4 - 1);
+        if (rp.gl[r] > rp.gr[r]) rp.live[r] = false;
+    }
+}
+
+struct ItemMap {
+    int first, stride, end;
+};
+
+// XCD-aware static item schedule of the persistent grid.
+__device__ __forceinline__ ItemMap item_map(int items) {
+    const int nb = gridDim.x;
+    const int xcd = blockIdx.x & 7;
+    const int per_xcd_blocks = (nb + 7 - xcd) >> 3;      // blocks on this XCD
+    const int local = blockIdx.x >> 3;
+    const int waves_here = per_xcd_blocks * (kNT / 64);
+    const int lo = (int)(((long long)items * xcd) >> 3);
+    const int hi = (int)(((long long)items * (xcd + 1)) >> 3);
+    ItemMap m;
+    m.first = lo + local * (kNT / 64) + (int)(threadIdx.x >> 6);
+    m.stride = waves_here;
+    m.end = hi;
+    return m;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// Density.  rho = m*poly6 * sum (h^2 - r^2)^3 + self_density over
+// 1e-12 <= r^2 <= h^2; w = 1/rho into pos.w, p = ((rho/rho0)^7 - 1)*gas into
+// ev.w (density_kernels.cu:353-358, 536-544).
+__device__ __forceinline__ float density_term(float4 a, float4 b, float h2) {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float t = h2 - d2;
+    return (d2 >= kFloatSmall && d2 <= h2) ? t * t * t : 0.f;
+}
+
+__device__ __forceinline__ void density_store(const PhysicsArgs &P, int i, float s) {
+    const gsph_constants &c = P.c;
+    const float rho = s * c.density_scale + c.self_density;
+    const float stored = rho < kFloatSmall ? c.rest_density : rho;
+    reinterpret_cast<float *>(&P.pos[i])[3] = 1.0f / stored;
+    reinterpret_cast<float *>(&P.ev[i])[3] =
+        (powf7(rho * c.inv_rest_density) - 1.f) * c.gas_constant;
+}
+
+template <bool kBatch>
+__global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
+    __shared__ float4 s_nb[kNT / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tra = P.dc[kCntTra];
+    const int T = P.dc[kCntT];
+    const int tra_tiles = (tra + 63) >> 6;
+    const int items = tra_tiles + ((T + 1) >> 1);
+    const ItemMap m = item_map(items);
+    const float h2 = P.c.kernel_2;
+    for (int it = m.first; it < m.end; it += m.stride) {
+        if (it < tra_tiles) {
+            const int idx = it * 64 + lane;
+            if (idx >= tra) continue;
+            const int i = P.compact[idx];
+            const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+            const float4 pi = P.pos[i];
+            int mx, my, mz;
+            micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
+            RowPlan rp;
+            plan_rows(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, P.row_prune != 0, rp);
+            float s = 0.f;
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                if (!rp.live[r]) continue;
+                int lo, hi;
+                row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r],
+                          rp.gr[r], lo, hi);
+                for (int j = lo; j < hi; ++j) s += density_term(pi, P.pos[j], h2);
+            }
+            density_store(P, i, s);
+        } else {
+            const int task = ((it - tra_tiles) << 1) + (lane >> 5);
+            const int hl = lane & 31;
+            const bool has_task = task < T;
+            gsph_task tk;
+            if (has_task) tk = P.tasks[task];
+            const int cell_end = has_task ? tk.cell_begin + tk.cell_particle_count : 0;
+            const int self = has_task ? tk.cell_begin + tk.p_offset + hl : 0;
+            const bool active = has_task && self < cell_end;
+            const int si = has_task ? min(self, cell_end - 1) : 0;
+            const float4 pi = P.pos[si];
+            int gidx = has_task ? scene_of_particle<kBatch>(P.scenes, P.nscenes, tk.cell_begin) : 0;
+            const SceneGeom g = P.scenes[gidx];
+            const int cx = has_task ? tk.cell_pos[0] : 0;
+            const int cy = has_task ? tk.cell_pos[1] : 0;
+            const int cz = has_task ? tk.cell_pos[2] : 0;
+            const int gl = 4 * cx - 4 + (has_task ? tk.xxi : 0);
+            const int gr = 4 * cx + 4 + (has_task ? tk.xxx : 0);
+            float s = 0.f;
+            float4 *lds = &s_nb[wave][lane & 32];
+            for (int r = 0; r < 9; ++r) {
+                const int ny = cy + r % 3 - 1, nz = cz + r / 3 - 1;
+                int lo = 0, hi = 0;
+                if (has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
+                    row_range(g, P.begin, P.xcum, cx, ny, nz, max(gl, 0),
+                              min(gr, g.gx * 4 - 1), lo, hi);
+                for (int b0 = lo; b0 < hi; b0 += 32) {
+                    const int cnt = min(32, hi - b0);
+                    if (hl < cnt) lds[hl] = P.pos[b0 + hl];
+                    wave_sync_lds();
+                    for (int k = 0; k < cnt; ++k) s += density_term(pi, lds[k], h2);
+                    wave_sync_lds();
+                }
+            }
+            if (active) density_store(P, self, s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Force (force_kernels.cu:525-561 summand, 818-852 / 1019-1051 epilogue).
+struct ForceAcc {
+    float px, py, pz, vx, vy, vz, gx, gy, gz, l;
+};
+
+__device__ __forceinline__ void force_term(const float4 &pi, const float4 &ei,
+                                           const float4 &pj, const float4 &ej,
+                                           const gsph_constants &c, ForceAcc &A) {
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    if (d2 < kFloatSmall || d2 > c.kernel_2) return;
+    const float inv = __frsqrt_rn(d2);
+    const float dist = d2 * inv;
+    const float V = pj.w;
+    const float kr = c.kernel_h - dist;
+    const float Vkr = V * kr;
+    const float pw = Vkr * (ei.w + ej.w) * kr * inv;
+    A.px -= rx * pw;
+    A.py -= ry * pw;
+    A.pz -= rz * pw;
+    A.vx += (ej.x - ei.x) * Vkr;
+    A.vy += (ej.y - ei.y) * Vkr;
+    A.vz += (ej.z - ei.z) * Vkr;
+    const float t = c.kernel_2 - d2;
+    const float Vt = V * t;
+    const float gw = Vt * t;
+    A.gx += rx * gw;
+    A.gy += ry * gw;
+    A.gz += rz * gw;
+    A.l += Vt * (d2 - 0.75f * t);
+}
+
+__device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, float wi,
+                                            ForceAcc A) {
+    const gsph_constants &c = P.c;
+    const float fx = A.px * c.half_spiky_value + A.vx * c.viscosity_visco_value;
+    const float fy = A.py * c.half_spiky_value + A.vy * c.viscosity_visco_value;
+    const float fz = A.pz * c.half_spiky_value + A.vz * c.viscosity_visco_value;
+    const float gx = A.gx * c.grad_color_scale;
+    const float gy = A.gy * c.grad_color_scale;
+    const float gz = A.gz * c.grad_color_scale;
+    const float l = A.l * c.lplc_color_scale * wi;
+    const float gn = sqrtf(gx * gx + gy * gy + gz * gz);
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    if (gn > c.surface_normal) {
+        const float k = c.surface_coe * l / gn;
+        sx = gx * k;
+        sy = gy * k;
+        sz = gz * k;
+    }
+    f3 a;
+    a.x = fx * c.mass + sx;
+    a.y = fy * c.mass + sy;
+    a.z = fz * c.mass + sz;
+    P.acc[i] = a;
+}
+
+template <bool kBatch>
+__global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
+    __shared__ float4 s_pos[kNT / 64][64];
+    __shared__ float4 s_ev[kNT / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tra = P.dc[kCntTra];
+    const int T = P.dc[kCntT];
+    const int tra_tiles = (tra + 63) >> 6;
+    const int items = tra_tiles + ((T + 1) >> 1);
+    const ItemMap m = item_map(items);
+    for (int it = m.first; it < m.end; it += m.stride) {
+        if (it < tra_tiles) {
+            const int idx = it * 64 + lane;
+            if (idx >= tra) continue;
+            const int i = P.compact[idx];
+            const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+            const float4 pi = P.pos[i];
+            const float4 ei = P.ev[i];
+            int mx, my, mz;
+            micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
+            RowPlan rp;
+            plan_rows(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, P.row_prune != 0, rp);
+            ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                if (!rp.live[r]) continue;
+                int lo, hi;
+                row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r],
+                          rp.gr[r], lo, hi);
+                for (int j = lo; j < hi; ++j) force_term(pi, ei, P.pos[j], P.ev[j], P.c, A);
+            }
+            force_store(P, i, pi.w, A);
+        } else {
+            const int task = ((it - tra_tiles) << 1) + (lane >> 5);
+            const int hl = lane & 31;
+            const bool has_task = task < T;
+            gsph_task tk;
+            if (has_task) tk = P.tasks[task];
+            const int cell_end = has_task ? tk.cell_begin + tk.cell_particle_count : 0;
+            const int self = has_task ? tk.cell_begin + tk.p_offset + hl : 0;
+            const bool active = has_task && self < cell_end;
+            const int si = has_task ? min(self, cell_end - 1) : 0;
+            const float4 pi = P.pos[si];
+            const float4 ei = P.ev[si];
+            int gidx = has_task ? scene_of_particle<kBatch>(P.scenes, P.nscenes, tk.cell_begin) : 0;
+            const SceneGeom g = P.scenes[gidx];
+            const int cx = has_task ? tk.cell_pos[0] : 0;
+            const int cy = has_task ? tk.cell_pos[1] : 0;
+            const int cz = has_task ? tk.cell_pos[2] : 0;
+            const int gl = 4 * cx - 4 + (has_task ? tk.xxi : 0);
+            const int gr = 4 * cx + 4 + (has_task ? tk.xxx : 0);
+            ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            float4 *lp = &s_pos[wave][lane & 32];
+            float4 *le = &s_ev[wave][lane & 32];
+            for (int r = 0; r < 9; ++r) {
+                const int ny = cy + r % 3 - 1, nz = cz + r / 3 - 1;
+                int lo = 0, hi = 0;
+                if (has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
+                    row_range(g, P.begin, P.xcum, cx, ny, nz, max(gl, 0),
+                              min(gr, g.gx * 4 - 1), lo, hi);
+                for (int b0 = lo; b0 < hi; b0 += 32) {
+                    const int cnt = min(32, hi - b0);
+                    if (hl < cnt) {
+                        lp[hl] = P.pos[b0 + hl];
+                        le[hl] = P.ev[b0 + hl];
+                    }
+                    wave_sync_lds();
+                    for (int k = 0; k < cnt; ++k) force_term(pi, ei, lp[k], le[k], P.c, A);
+                    wave_sync_lds();
+                }
+            }
+            if (active) force_store(P, self, pi.w, A);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// knIntegrateVelocityE (src/solver/integration_kernels.cu:235-336): penalty
+// walls, a*w + g, |a| and |v| clamps, Euler step, wall clamps, leapfrog ev.
+// Reads the sorted buffer, writes the resident one.
+template <bool kBatch>
+__global__ __launch_bounds__(kNT) void k_integrate(PhysicsArgs P) {
+    const int i = blockIdx.x * kNT + threadIdx.x;
+    if (i >= P.n) return;
+    const gsph_constants &c = P.c;
+    const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+    const float4 p = P.pos[i];
+    const float4 e = P.ev[i];
+    const f3 vv = P.vel[i];
+    const f3 aa = P.acc[i];
+    float v[3] = {vv.x, vv.y, vv.z};
+    float a[3] = {aa.x, aa.y, aa.z};
+    float x[3] = {p.x, p.y, p.z};
+    const float alpha = 20000000.f, beta = 200000.f;
+    const float kern = (float)((double)c.kernel_h + 0.01);
+    const int order[3] = {1, 0, 2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int ax = order[k];
+        float diff = kern - (x[ax] - g.bound_min[ax]);
+        if (diff > 0.f) a[ax] += alpha * diff - beta * v[ax];
+        diff = kern - (g.bound_max[ax] - x[ax]);
+        if (diff > 0.f) a[ax] -= alpha * diff + beta * v[ax];
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) a[ax] = a[ax] * p.w + c.gravity[ax];
+    float sp = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    if (sp > 3000000.f) {
+        const float f = sqrtf(3000000.f / sp);
+        a[0] *= f;
+        a[1] *= f;
+        a[2] *= f;
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) v[ax] += a[ax] * c.time_step;
+    sp = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (sp > 36.f) {
+        const float f = sqrtf(36.f / sp);
+        v[0] *= f;
+        v[1] *= f;
+        v[2] *= f;
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) x[ax] += v[ax] * c.time_step;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        if (x[ax] >= g.bound_max[ax]) {
+            v[ax] *= c.wall_damping;
+            x[ax] = g.bound_max[ax] - kern;
+        }
+        if (x[ax] <= g.bound_min[ax]) {
+            v[ax] *= c.wall_damping;
+            x[ax] = g.bound_min[ax] + kern;
+        }
+    }
+    P.pos_out[i] = make_float4(x[0], x[1], x[2], p.w);
+    f3 vo;
+    vo.x = v[0];
+    vo.y = v[1];
+    vo.z = v[2];
+    P.vel_out[i] = vo;
+    P.ev_out[i] = make_float4((v[0] + e.x) * 0.5f, (v[1] + e.y) * 0.5f,
+                              (v[2] + e.z) * 0.5f, e.w);
+}
+
+// knInit (integration_kernels.cu:6-17): ev = 0, acc = 0.
+__global__ void k_init_state(float4 *ev, f3 *acc, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ev[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    f3 z;
+    z.x = z.y = z.z = 0.f;
+    acc[i] = z;
+}
+
+// FP64 checksum: block b sums slots [b*chunk, (b+1)*chunk) in a fixed tree
+// order; the final block adds the partials in block order.
+constexpr int kSumBlocks = 256;
+
+__global__ __launch_bounds__(kNT) void k_checksum_partial(const float4 *pos, int n,
+                                                          double *partial) {
+    __shared__ double s[5][kNT];
+    const int chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int b0 = blockIdx.x * chunk;
+    const int b1 = min(n, b0 + chunk);
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int i = b0 + threadIdx.x; i < b1; i += kNT) {
+        const float4 q = pos[i];
+        acc[0] += q.x;
+        acc[1] += q.y;
+        acc[2] += q.z;
+        acc[3] += q.w;
+        acc[4] += (double)((i & 1023) + 1) * ((double)q.x + 3.0 * q.y + 7.0 * q.z);
+    }
+    for (int k = 0; k < 5; ++k) s[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = kNT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 5; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) partial[blockIdx.x * 5 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void k_checksum_final(const double *partial, int nblocks, double *out) {
+    if (threadIdx.x >= 5) return;
+    double t = 0;
+    for (int b = 0; b < nblocks; ++b) t += partial[b * 5 + threadIdx.x];
+    out[threadIdx.x] = t;
+}
+
+}  // namespace
+
+void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s) {
+    if (batch)
+        hipLaunchKernelGGL(k_density<true>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    else
+        hipLaunchKernelGGL(k_density<false>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+}
+
+void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s) {
+    if (batch)
+        hipLaunchKernelGGL(k_force<true>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    else
+        hipLaunchKernelGGL(k_force<false>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+}
+
+void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s) {
+    const int blocks = (p.n + kNT - 1) / kNT;
+    if (blocks == 0) return;
+    if (batch)
+        hipLaunchKernelGGL(k_integrate<true>, dim3(blocks), dim3(kNT), 0, s, p);
+    else
+        hipLaunchKernelGGL(k_integrate<false>, dim3(blocks), dim3(kNT), 0, s, p);
+}
+
+void launch_checksum(const float4 *pos, int n, double *partial, int nblocks,
+                     double *out5, hipStream_t s) {
+    (void)nblocks;
+    hipLaunchKernelGGL(k_checksum_partial, dim3(kSumBlocks), dim3(kNT), 0, s, pos, n, partial);
+    hipLaunchKernelGGL(k_checksum_final, dim3(1), dim3(64), 0, s, partial, kSumBlocks, out5);
+}
+
+void launch_init_state(float4 *ev, f3 *acc, int n, hipStream_t s) {
+    const int blocks = (n + kNT - 1) / kNT;
+    if (blocks == 0) return;
+    hipLaunchKernelGGL(k_init_state, dim3(blocks), dim3(kNT), 0, s, ev, acc, n);
+}
+
+}  // namespace gsph

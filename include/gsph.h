@@ -1,0 +1,238 @@
+/*
+ * gsph.h - C ABI of the MI355X-native hybrid SPH step.
+ *
+ * The reference has no plugin or FFI layer: its hot path sits behind the
+ * in-process C++ classes sph::HybridSystem (src/simulation/sph_hybrid_system.h:46-137)
+ * and sph::Arrangement (src/grid/sph_arrangement.h:28-166) plus the solver
+ * wrappers in src/solver/kernel_dispatch.cuh:22-28,67-83,106.  Each entry point
+ * below replaces one of those seams; the comment on each names the reference
+ * interface it stands in for.
+ *
+ * Conventions
+ *   - every function returns an int status: GSPH_OK (0) or a negative GSPH_ERR_*;
+ *     nothing in the library calls exit() (the reference's CUDA_SAFE_CALL does,
+ *     src/core/cuda_call_check.h:15-39).
+ *   - gsph_last_error(ctx) returns the message of the last failure on that ctx
+ *     (gsph_last_error(NULL) the last failure that had no ctx).
+ *   - host pointers are plain arrays in host memory; particle arrays are
+ *     interleaved: pos4 = {x,y,z,w}*n with w = 1/density, vel3 = {x,y,z}*n,
+ *     ev4 = {x,y,z,p}*n with p = pressure, acc3 = {x,y,z}*n.  This is the
+ *     reference's ParticleBufferList SoA (src/particle/particle_buffer.h:116-172).
+ *   - a ctx owns all of its device memory and one non-blocking HIP stream of
+ *     default priority; calls on one ctx must be externally serialised, distinct
+ *     ctx may be used from distinct threads.
+ */
+#ifndef GSPH_H_
+#define GSPH_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSPH_OK 0
+#define GSPH_ERR_INVALID_ARG (-1)
+#define GSPH_ERR_HIP (-2)
+#define GSPH_ERR_OUT_OF_DOMAIN (-3)
+#define GSPH_ERR_CAPACITY (-4)
+#define GSPH_ERR_NO_DEVICE (-5)
+#define GSPH_ERR_SPIN_TIMEOUT (-6)
+#define GSPH_ERR_IO (-7)
+#define GSPH_ERR_BUDGET (-8)
+#define GSPH_ERR_INJECTED (-9)
+
+/* Physics inputs of a scene.  The reference hard-codes these in
+ * defaultInitializeSPHSysPara (src/simulation/sph_hybrid_system.cpp:127-202);
+ * here they are inputs, with gsph_params_default() returning the reference's
+ * values (h = 0.03, g = -9.8, dt = 0.003, ...). */
+typedef struct gsph_params {
+    float kernel_h;       /* smoothing length h; also the grid cell size */
+    float mass;           /* particle mass (one global mass, as the reference) */
+    float rest_density;
+    float gas_constant;
+    float viscosity;
+    float time_step;
+    float gravity[3];
+    float wall_damping;
+    float surface_normal; /* |grad colour| threshold for surface tension */
+    float surface_coe;
+    float world_size[3];  /* box extent; the grid is ceil(world/h) cells per axis */
+} gsph_params;
+
+/* Derived constants, the FP32 values the kernels use (the reference's
+ * SystemParameter, src/core/sph_parameter.h:18-90, derived as in
+ * sph_hybrid_system.cpp:143-193). */
+typedef struct gsph_constants {
+    float kernel_h, kernel_2, cell_size, inv_cell_size;
+    int grid_size[3];
+    float mass, rest_density, gas_constant, viscosity, time_step;
+    float gravity[3];
+    float wall_damping, surface_normal, surface_coe;
+    float poly6_value, spiky_value, visco_value, grad_poly6, lplc_poly6;
+    float self_density;
+    float inv_rest_density, density_scale, half_spiky_value;
+    float viscosity_visco_value, grad_color_scale, lplc_color_scale;
+    float bound_min[3], bound_max[3];
+    float world_size[3];
+} gsph_constants;
+
+/* 32-byte SMS task descriptor, the layout of the reference's BlockTask
+ * (src/particle/particle_buffer.h:60-95). */
+typedef struct gsph_task {
+    int8_t is_same;
+    uint8_t pad0;
+    uint16_t paired_bounds; /* 2 bits each: minx,maxx,miny,maxy,minz,maxz */
+    int32_t cellid;
+    uint16_t cell_pos[3];
+    uint16_t p_offset;
+    int8_t xxi, xxx, yyi, yyy, zzi, zzz;
+    int8_t pad1[2];
+    int32_t cell_begin;
+    int32_t cell_particle_count;
+} gsph_task;
+
+/* The reference's SmsTaskPairStats (src/grid/sph_arrangement.h:17-26). */
+typedef struct gsph_pair_stats {
+    int total_pairs;
+    int same_cell_pairs;
+    int is_same_pairs;
+    int full_is_same_pairs;
+    int partial_is_same_pairs;
+    int compact_is_same_pairs;
+    int widened_is_same_pairs;
+} gsph_pair_stats;
+
+/* Per-stage device times of one step in ms, from HIP events on the ctx stream
+ * (the reference's tick_events_, sph_hybrid_system.cpp:266-333). */
+typedef struct gsph_stage_ms {
+    float arrange;
+    float density;
+    float force;
+    float integrate;
+    float total;
+} gsph_stage_ms;
+
+/* Scheduling options.  split_mode 0 = the reference split heuristic
+ * (src/io/gpu_model.cu:172-180, bit-exact task assignment); 1 = tuned
+ * thresholds below (same physics, different TRA/SMS assignment).  The
+ * remaining fields are launch-shape knobs chosen by gsph_autotune. */
+typedef struct gsph_options {
+    int split_mode;
+    int sms_min_self;     /* tuned: cell is TRA when n <  sms_min_self ... */
+    int sms_min_total;    /* ... and the 7-cell total < sms_min_total */
+    int blocks_per_cu;    /* persistent grid size of density/force, per CU */
+    int sms_batch;        /* LDS staging depth of the SMS path: 32 or 64 */
+    int row_prune;        /* 1: skip rows/layers farther than h (exact superset) */
+} gsph_options;
+
+typedef struct gsph_ctx gsph_ctx;
+
+/* Library ------------------------------------------------------------------ */
+const char *gsph_version(void);
+int gsph_device_count(int *count);
+const char *gsph_last_error(const gsph_ctx *ctx);
+void gsph_params_default(gsph_params *out);
+void gsph_options_default(gsph_options *out);
+/* Replaces defaultInitializeSPHSysPara (sph_hybrid_system.cpp:127-202). */
+int gsph_derive_constants(const gsph_params *p, gsph_constants *out);
+
+/* Scenes (host only) ------------------------------------------------------- */
+/* Reads the reference scene JSON (assets/scene_default.json format; reader at
+ * sph_hybrid_system.cpp:69-126) and generates its lattice exactly as
+ * initializeScene does (sph_hybrid_system.cpp:432-473).  Optional extension
+ * keys: kernel_h, gravity_y, time_step, rest_density.  Call with pos4 = NULL
+ * to get the count, then again with a buffer of 4*count floats. */
+int gsph_scene_json(const char *path, gsph_params *params, int *count,
+                    float *pos4, int capacity);
+/* Synthetic workloads: 0 = 32^3 dam-break block, 1 = 158^3 column in a
+ * 6.32 m cube, 2 = pool plus fine sphere.  Same two-call protocol. */
+int gsph_scene_config(int config, gsph_params *params, int *count,
+                      float *pos4, int capacity);
+/* One jittered lattice block (the building block of the synthetic scenes):
+ * nx*ny*nz particles at corner + (i+1/2)*spacing, jitter +-jitter_frac*spacing
+ * from SplitMix64(seed). */
+int gsph_scene_block(const float corner[3], int nx, int ny, int nz,
+                     float spacing, float jitter_frac, uint64_t seed,
+                     float *pos4);
+
+/* Batched form --------------------------------------------------------------- */
+/* No counterpart in the reference (it runs one scene per process).  nscenes
+ * independent scenes share one context: scene s has box world3[3s..3s+2] and
+ * counts[s] particles, stored back to back in pos4 / vel3 (vel3 may be NULL).
+ * The physics constants (h, mass, rho0, dt, ...) are shared; p->world_size is
+ * ignored.  Every gsph_* step call then advances all scenes with one launch
+ * per kernel.  Scenes never interact, and each scene's particles keep the
+ * slot range [first, first + count) given by gsph_scene_range. */
+int gsph_batch_create(const gsph_params *p, const gsph_options *opt, int nscenes,
+                      const float *world3, const int *counts, const float *pos4,
+                      const float *vel3, gsph_ctx **out);
+int gsph_scene_count(const gsph_ctx *ctx, int *nscenes);
+int gsph_scene_range(const gsph_ctx *ctx, int scene, int *first, int *count);
+
+/* Plain single scene -------------------------------------------------------- */
+/* HybridSystem ctor + initializeScene + BuffInit (sph_hybrid_system.cpp:209-240,
+ * 432-473; kernel_dispatch.cu:17-22): uploads n particles, ev = acc = 0. */
+int gsph_create(const gsph_params *p, const gsph_options *opt, int n,
+                const float *pos4, const float *vel3, gsph_ctx **out);
+int gsph_destroy(gsph_ctx *ctx);
+int gsph_num_particles(const gsph_ctx *ctx, int *n);
+int gsph_get_constants(const gsph_ctx *ctx, gsph_constants *out);
+/* Overwrites the device state (any of vel3/ev4 may be NULL = zero). */
+int gsph_set_state(gsph_ctx *ctx, const float *pos4, const float *vel3,
+                   const float *ev4);
+/* Arrangement::arrangeHybridFrame (src/grid/sph_arrangement.cu:1493-1506):
+ * hash, stable sort, permute, cell begin/count, split, compact index, tasks,
+ * pairing.  No host synchronisation. */
+int gsph_arrange(gsph_ctx *ctx);
+/* computeDensityHybrid (src/solver/kernel_dispatch.cu:206-222). */
+int gsph_density(gsph_ctx *ctx);
+/* computeForceHybrid (src/solver/kernel_dispatch.cu:224-253). */
+int gsph_force(gsph_ctx *ctx);
+/* advance -> knIntegrateVelocityE (kernel_dispatch.cu:314-323). */
+int gsph_integrate(gsph_ctx *ctx);
+/* HybridSystem::tick x nsteps (sph_hybrid_system.cpp:253-339).  When
+ * per_step_ms is non-NULL it receives nsteps stage records (events per step);
+ * otherwise the steps run back to back with no host synchronisation. */
+int gsph_step(gsph_ctx *ctx, int nsteps, gsph_stage_ms *per_step_ms);
+/* Captures one full step as a hipGraph; later gsph_step calls replay it. */
+int gsph_enable_graph(gsph_ctx *ctx, int enable);
+int gsph_sync(gsph_ctx *ctx);
+/* Device -> host copies of the current state (sorted order).  NULL skips. */
+int gsph_read_state(gsph_ctx *ctx, float *pos4, float *vel3, float *ev4,
+                    float *acc3);
+/* Sizes of the grid outputs: cells = C, tasks = T (SMS task count),
+ * middle_raw = the reference's d_middle_value_ (-1 when there is no SMS
+ * particle), tra = TRA particle count (middle clamped to [0, n]). */
+int gsph_grid_info(gsph_ctx *ctx, int *cells, int *tasks, int *middle_raw,
+                   int *tra);
+/* Arrangement getters (sph_arrangement.h:41-57): sorted micro hash[n], the
+ * stable permutation perm[n] (sorted slot -> previous slot), cell begin[C] and
+ * count[C] (getDevCellOffset/getDevCellNumP), SMS tasks per cell req[C]
+ * (d_block_reqs_), compact[n] (getDevCellIndex), tasks[T+1] (getSmsTasks,
+ * including the odd-tail pad).  NULL skips an array. */
+int gsph_read_grid(gsph_ctx *ctx, int *hash, int *perm, int *begin,
+                   int *count, int *req, int *compact, gsph_task *tasks);
+/* computeBenchmarkStateChecksum (sph_hybrid_system.cpp:43-62): sum x, y, z,
+ * w and sum ((i&1023)+1)(x+3y+7z) in FP64, reduced on the device in a fixed
+ * order. */
+int gsph_checksum(gsph_ctx *ctx, double out[5]);
+/* The same checksum over state slots [first, first + count), with i counted
+ * from first (the checksum of one scene of a batch). */
+int gsph_checksum_range(gsph_ctx *ctx, int first, int count, double out[5]);
+/* Arrangement::getSmsTaskPairStats (sph_arrangement.cu:1595-1636). */
+int gsph_get_pair_stats(gsph_ctx *ctx, gsph_pair_stats *out);
+/* Per-kernel device time (ms, summed) accumulated while kernel timing is on;
+ * names are written as a ';'-separated list. */
+int gsph_kernel_timing(gsph_ctx *ctx, int enable);
+int gsph_kernel_times(gsph_ctx *ctx, char *names, int names_cap, float *ms,
+                      int *launches, int cap, int *count);
+/* Test hook: the next call that launches kernels fails with GSPH_ERR_INJECTED
+ * after launching nothing (failure-path tests of the serving loop). */
+int gsph_inject_fault(gsph_ctx *ctx, int after_calls);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSPH_H_ */

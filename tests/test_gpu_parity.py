@@ -1,0 +1,358 @@
+"""GPU parity tests: the HIP path through the C ABI against the CPU oracle.
+
+Bars:
+  * arrange outputs (sorted micro hash, stable permutation, cell begin/count,
+    split req, compact index, middle value, SMS tasks and pairing) are
+    bit-exact;
+  * density (1/rho), pressure, acceleration, integrated position and velocity
+    after one step are within REL_TOL = 1e-5 of the FP64 oracle, each relative
+    to its own scale (see tolerance.py);
+  * the state checksum after 10 steps agrees with the FP32 CPU reference step
+    within CHECKSUM_TOL.
+"""
+import numpy as np
+import pytest
+
+from helpers import cfg0_params, dense_scene, make_params, mixed_scene, to_gsph_params
+from tolerance import REL_TOL, step_errors
+
+pytestmark = pytest.mark.gpu
+
+CHECKSUM_TOL = 2e-6
+TASK_FIELDS = ("is_same", "paired_bounds", "cellid", "cell_pos", "p_offset", "xxi", "xxx",
+               "yyi", "yyy", "zzi", "zzz", "cell_begin", "cell_particle_count")
+PAD_FIELDS = ("is_same", "cellid", "cell_pos", "p_offset", "cell_begin", "cell_particle_count")
+
+
+def banner(msg):
+    print(f"\n[gpu-case] {msg}", flush=True)
+
+
+def scene(po, name):
+    if name == "cfg0":
+        return cfg0_params(po)
+    if name == "dense":
+        return dense_scene(po)
+    if name == "mixed":
+        return mixed_scene(po)
+    if name == "odd_tasks":
+        # one cell row of dense cells whose task count is odd
+        # a dense block (20 tasks) plus one isolated cell of 20 particles
+        # (req (20 + 27) >> 5 = 1): 21 tasks, so judgeTask pads the tail
+        p = make_params(h=0.02, world=(0.2, 0.2, 0.2))
+        blk = po.scene_block([0.06, 0.06, 0.06], 9, 8, 8, 0.005, 0.01, 9)
+        rng = np.random.default_rng(9)
+        extra = np.zeros((20, 4), np.float32)
+        extra[:, :3] = rng.uniform(0.1405, 0.1595, (20, 3))
+        return p, np.concatenate([blk, extra])
+    if name == "sparse_random":
+        rng = np.random.default_rng(4)
+        p = make_params(h=0.02, world=(0.5, 0.3, 0.4))
+        pos = np.zeros((5000, 4), np.float32)
+        pos[:, :3] = rng.uniform([0.031] * 3, [0.469, 0.269, 0.369], (5000, 3))
+        return p, pos
+    raise KeyError(name)
+
+
+SCENES = ["cfg0", "dense", "mixed", "odd_tasks", "sparse_random"]
+
+
+def make_system(gsph, op, pos, **opts):
+    o = gsph.options_default()
+    for k, v in opts.items():
+        setattr(o, k, v)
+    return gsph.System(to_gsph_params(gsph, op), pos, options=o)
+
+
+def compare_grid(g, ref):
+    for k in ("hash", "perm", "begin", "count", "req", "compact"):
+        assert np.array_equal(g[k], ref[k]), k
+    assert g["middle_raw"] == ref["middle_raw"]
+    assert g["tasks"] is not None and len(g["tasks"]) == len(ref["tasks"])
+    T = ref["tasks_count"]
+    assert g["tasks"].shape[0] == T + (T & 1)
+    for f in TASK_FIELDS:
+        assert np.array_equal(g["tasks"][f][:T], ref["tasks"][f][:T]), f
+    if T & 1:
+        for f in PAD_FIELDS:
+            assert np.array_equal(g["tasks"][f][T], ref["tasks"][f][T]), "pad " + f
+    tra = ref["middle_raw"] if ref["middle_raw"] >= 0 else len(ref["perm"])
+    assert g["tra"] == tra
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_arrange_bit_exact(gsph, po, name):
+    op, pos = scene(po, name)
+    banner(f"arrange {name} n={len(pos)}")
+    s = make_system(gsph, op, pos)
+    s.arrange()
+    g = s.read_grid()
+    ref = po.arrange(op, pos)
+    compare_grid(g, ref)
+    if name in ("dense", "mixed", "odd_tasks"):
+        assert ref["tasks_count"] > 0
+    if name == "odd_tasks":
+        assert ref["tasks_count"] & 1
+    s.close()
+
+
+def test_arrange_reference_default_scene(gsph, po):
+    """The reference's own scene (3,944,312 particles, ~296 per cell, SMS)."""
+    from pathlib import Path
+    path = Path(__file__).resolve().parent.parent / "fast-general-gpu-sph-framework_amd" / \
+        "assets" / "scene_default.json"
+    p, pos = gsph.scene_json(path)
+    banner(f"arrange scene_default n={len(pos)}")
+    s = gsph.System(p, pos)
+    s.arrange()
+    g = s.read_grid()
+    import pyoracle
+    ref = po.arrange(pyoracle.params_from(p), pos)
+    compare_grid(g, ref)
+    stats = s.pair_stats()
+    T = ref["tasks_count"]
+    assert stats["total_pairs"] == (T + 1) // 2 and T > 100000
+    s.close()
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_one_step_physics(gsph, po, name):
+    op, pos = scene(po, name)
+    n = len(pos)
+    banner(f"step {name} n={n}")
+    s = make_system(gsph, op, pos)
+    s.step(1)
+    pos1, vel1, ev1, acc1 = s.read_state()
+    ref = po.fp64_step(op, pos, np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32))
+    errs = step_errors(pos1, vel1, ev1, acc1, ref, op)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e} > {REL_TOL}"
+    s.close()
+
+
+def test_step_from_moving_state(gsph, po):
+    """Physics parity on a state with velocities (viscosity active): run the
+    oracle from the GPU's own state after 5 steps."""
+    op, pos = mixed_scene(po)
+    banner("moving state mixed")
+    s = make_system(gsph, op, pos)
+    s.step(5)
+    p5, v5, e5, _ = s.read_state()
+    s.step(1)
+    pos1, vel1, ev1, acc1 = s.read_state()
+    ref = po.fp64_step(op, p5, v5, e5)
+    errs = step_errors(pos1, vel1, ev1, acc1, ref, op)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e} > {REL_TOL}"
+    s.close()
+
+
+def test_ten_step_checksum_and_grid(gsph, po):
+    op, pos = cfg0_params(po)
+    banner("10 steps cfg0")
+    s = make_system(gsph, op, pos)
+    s.step(10)
+    pg, vg, eg, _ = s.read_state()
+    cs = s.checksum()
+    # device FP64 reduction == host sequential FP64 sum of the same state
+    np.testing.assert_allclose(cs, po.checksum(pg), rtol=1e-12)
+    P, V, E, A = po.step_f32(op, pos, nsteps=10)
+    ref = po.checksum(P)
+    rel = np.abs(cs - ref) / np.abs(ref)
+    print("checksum rel diff", rel, flush=True)
+    assert rel.max() <= CHECKSUM_TOL
+    # the grid stays bit-exact on the GPU's own trajectory
+    s.arrange()
+    compare_grid(s.read_grid(), po.arrange(op, pg))
+    s.close()
+
+
+def test_determinism_and_prune_equivalence(gsph, po):
+    op, pos = mixed_scene(po)
+    banner("determinism mixed")
+    out = []
+    for prune in (1, 1, 0):
+        s = make_system(gsph, op, pos, row_prune=prune)
+        s.step(3)
+        out.append(s.read_state())
+        s.close()
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k])     # run to run
+        assert np.array_equal(out[0][k], out[2][k])     # pruned rows change nothing
+
+
+def test_graph_replay_matches_eager(gsph, po):
+    op, pos = dense_scene(po)
+    banner("graph replay dense")
+    a = make_system(gsph, op, pos)
+    a.step(4)
+    ref = a.read_state()
+    b = make_system(gsph, op, pos)
+    b.enable_graph(True)
+    for _ in range(4):          # replays with host work and syncs in between
+        b.step(1)
+        b.sync()
+    got = b.read_state()
+    for k in range(4):
+        assert np.array_equal(ref[k], got[k])
+    a.close()
+    b.close()
+
+
+def test_stage_timings_and_kernel_times(gsph, po):
+    op, pos = cfg0_params(po)
+    s = make_system(gsph, op, pos)
+    t = s.step(2, timings=True)
+    assert len(t) == 2 and all(x["total"] > 0 for x in t)
+    s.kernel_timing(True)
+    s.step(2)
+    kt = s.kernel_times()
+    assert kt["force"][1] == 2 and kt["density"][1] == 2 and kt["hash"][1] == 2
+    s.kernel_timing(False)
+    s.close()
+
+
+def test_tuned_split_same_physics(gsph, po):
+    op, pos = mixed_scene(po)
+    n = len(pos)
+    banner("tuned split mixed")
+    s = make_system(gsph, op, pos, split_mode=1, sms_min_self=9, sms_min_total=64)
+    s.arrange()
+    T_tuned = s.grid_info()["tasks"]
+    s.density()
+    s.force()
+    s.integrate()
+    pos1, vel1, ev1, acc1 = s.read_state()
+    ref = po.fp64_step(op, pos, np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32))
+    assert T_tuned != ref["grid"]["tasks_count"]
+    errs = step_errors(pos1, vel1, ev1, acc1, ref, op)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e}"
+    s.close()
+
+
+def test_edge_sizes(gsph, po):
+    op = make_params(h=0.02, world=(0.2, 0.2, 0.2))
+    banner("edge sizes")
+    for pts in ([[0.1, 0.1, 0.1, 0]], [[0.1, 0.1, 0.1, 0], [0.11, 0.1, 0.1, 0]]):
+        pos = np.array(pts, np.float32)
+        s = make_system(gsph, op, pos)
+        s.arrange()
+        compare_grid(s.read_grid(), po.arrange(op, pos))
+        s.density()
+        s.force()
+        s.integrate()
+        p1, v1, e1, a1 = s.read_state()
+        ref = po.fp64_step(op, pos, np.zeros((len(pos), 3), np.float32),
+                           np.zeros((len(pos), 4), np.float32))
+        assert np.abs(p1[:, 3] * ref["rho"] - 1).max() <= REL_TOL
+        s.close()
+    # an empty scene steps and reports zeros
+    s = make_system(gsph, op, np.zeros((0, 4), np.float32))
+    s.step(2)
+    s.arrange()
+    info = s.grid_info()
+    assert info["tasks"] == 0 and info["middle_raw"] == -1
+    g = s.read_grid()
+    assert g["count"].sum() == 0 and np.all(g["begin"] == 0)
+    s.close()
+
+
+def test_out_of_domain_is_reported(gsph, po):
+    op = make_params(h=0.02, world=(0.2, 0.2, 0.2))
+    pos = po.scene_block([0.06, 0.06, 0.06], 4, 4, 4, 0.01, 0.0, 1)
+    pos[5, 0] = 0.35            # outside the box
+    s = make_system(gsph, op, pos)
+    s.arrange()
+    with pytest.raises(gsph.GsphError) as e:
+        s.sync()
+    assert e.value.code == -3
+    pos[5, 0] = np.nan
+    s.set_state(pos)
+    s.arrange()
+    with pytest.raises(gsph.GsphError) as e:
+        s.sync()
+    assert e.value.code == -3
+    s.close()
+
+
+def test_api_misuse_and_injected_fault(gsph, po):
+    op, pos = dense_scene(po)
+    s = make_system(gsph, op, pos)
+    with pytest.raises(gsph.GsphError):
+        s.density()                     # no grid yet
+    s.inject_fault(1)
+    s.arrange()                         # call 1 passes
+    with pytest.raises(gsph.GsphError) as e:
+        s.density()                     # call 2 fails, launches nothing
+    assert e.value.code == -9
+    s.density()
+    s.close()
+
+
+def test_batch_alone_vs_mix(gsph, po):
+    """Scenes batched into one context give bitwise the results they give alone."""
+    rng = np.random.default_rng(21)
+    op = make_params(h=0.02, world=(1, 1, 1))
+    worlds, chunks = [], []
+    for k in range(5):
+        nx, ny, nz = rng.integers(4, 14, 3)
+        pos = po.scene_block([0.06, 0.06, 0.06], int(nx), int(ny), int(nz), 0.01, 0.01, 100 + k)
+        if k == 2:   # a dense scene (SMS tasks)
+            pos = po.scene_block([0.06, 0.06, 0.06], 12, 12, 12, 0.005, 0.01, 100 + k)
+        ext = pos[:, :3].max(0) - 0.06 + 0.01
+        worlds.append([max(0.2, 2 * e + 0.12) for e in ext])
+        chunks.append(pos)
+    counts = [len(c) for c in chunks]
+    gp = to_gsph_params(gsph, op)
+    banner(f"batch alone vs mix counts={counts}")
+    b = gsph.System(gp, np.concatenate(chunks), batch_worlds=worlds, batch_counts=counts)
+    b.step(3)
+    bp, bv, be, ba = b.read_state()
+    for k, c in enumerate(chunks):
+        gp1 = to_gsph_params(gsph, op)
+        for a in range(3):
+            gp1.world_size[a] = worlds[k][a]
+        s = gsph.System(gp1, c)
+        s.step(3)
+        sp, sv, se, sa = s.read_state()
+        f, cnt = b.scene_range(k)
+        assert cnt == len(c)
+        for x, y in ((bp, sp), (bv, sv), (be, se), (ba, sa)):
+            assert np.array_equal(x[f:f + cnt], y)
+        cs_b = b.checksum(f, cnt)
+        assert np.array_equal(cs_b, s.checksum())
+        s.close()
+    b.close()
+
+
+def test_config1_full_size_properties(gsph, po):
+    """Workload 1 at full size (3,944,312 particles, 317^3 cells): properties
+    that do not depend on the oracle fitting in memory."""
+    p, pos = gsph.scene_config(1)
+    n = len(pos)
+    banner(f"config1 n={n}")
+    s = gsph.System(p, pos)
+    s.arrange()
+    g = s.read_grid()
+    perm = g["perm"]
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    import pyoracle
+    key, bad = po.hash_(pyoracle.params_from(p), pos)
+    assert bad == 0
+    assert np.array_equal(g["hash"], key[perm])
+    assert np.array_equal(perm, np.lexsort((np.arange(n), key)))
+    cnt = g["count"]
+    assert cnt.sum() == n
+    assert np.array_equal(g["begin"][1:], np.cumsum(cnt)[:-1]) and g["begin"][0] == 0
+    assert np.array_equal(g["compact"], np.arange(n))      # all TRA at 8 per cell
+    assert g["tasks"].shape[0] == 0 and g["middle_raw"] == -1
+    s.density()
+    s.force()
+    s.integrate()
+    s.sync()
+    cs = s.checksum()
+    assert np.all(np.isfinite(cs))
+    s.close()
