@@ -51,6 +51,109 @@ def cpu_baseline(params, pos, steps, threads):
     }
 
 
+def batch_scenes(gsph, count=256, seed=20261206):
+    """Workload 3: `count` dam-break blocks, N log-uniform in [4096, 262144],
+    aspect ratios y/x and z/x uniform in [0.5, 2], spacing 0.01, corner 0.06,
+    box 2x the block extent per axis."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    worlds, blocks = [], []
+    for k in range(count):
+        n_target = float(np.exp(rng.uniform(np.log(4096), np.log(262144))))
+        ay, az = rng.uniform(0.5, 2.0, 2)
+        nx = max(4, int(round((n_target / (ay * az)) ** (1 / 3))))
+        ny, nz = max(4, int(round(ay * nx))), max(4, int(round(az * nx)))
+        pos = gsph.scene_block([0.06, 0.06, 0.06], nx, ny, nz, 0.01, 0.01, seed + 7919 * (k + 1))
+        worlds.append([2 * nx * 0.01, 2 * ny * 0.01, 2 * nz * 0.01])
+        blocks.append(pos)
+    return worlds, blocks
+
+
+def batched_leg(gsph, steps, warmup, count=256, loop_steps=5):
+    import numpy as np
+
+    params, _ = gsph.scene_config(0)
+    worlds, blocks = batch_scenes(gsph, count)
+    counts = [len(b) for b in blocks]
+    n = int(sum(counts))
+    b = gsph.System(params, np.concatenate(blocks), batch_worlds=worlds, batch_counts=counts)
+    b.enable_graph(True)
+    b.step(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    batched_ps = n * steps / wall
+    per_step_ms = wall * 1e3 / steps
+    b.close()
+    # the plain per-scene launch loop: one context (own stream) per scene
+    systems = []
+    for w, blk in zip(worlds, blocks):
+        p, _ = gsph.scene_config(0)
+        for a in range(3):
+            p.world_size[a] = w[a]
+        systems.append(gsph.System(p, blk))
+    for s in systems:
+        s.step(1)
+    for s in systems:
+        s.sync()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for s in systems:
+            s.step(1)
+    for s in systems:
+        s.sync()
+    loop_wall = time.perf_counter() - t0
+    loop_ps = n * loop_steps / loop_wall
+    for s in systems:
+        s.close()
+    return {
+        "workload": f"{count} dam-break scenes, N log-uniform 4096..262144 "
+                    f"(total {n}), one context, hipGraph step",
+        "scenes": count, "particles": n,
+        "particle_steps_per_s": batched_ps, "scenes_per_s": count * steps / wall,
+        "ms_per_step": per_step_ms,
+        "plain_loop_particle_steps_per_s": loop_ps,
+        "plain_loop_scenes_per_s": count * loop_steps / loop_wall,
+        "speedup_vs_plain_loop": batched_ps / loop_ps,
+    }
+
+
+def serving_leg(gsph, capacity_ps, requests=96, seed=20261207, budget=200e9):
+    """Workload 4: Poisson arrivals, N log-uniform in [8192, 1048576], steps
+    uniform in 50..500, 200 GB budget; the arrival rate puts the offered load
+    at 0.7 of the measured batched capacity."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(8192), np.log(1048576), requests))
+    steps = rng.integers(50, 501, requests)
+    mean_work = float(np.mean(ns) * np.mean(steps))
+    lam = 0.7 * capacity_ps / mean_work
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
+    params, _ = gsph.scene_config(0)
+    srv = gsph.Server(params, hbm_budget_bytes=budget, max_scenes=4096)
+    for k in range(requests):
+        side = max(8, int(round(ns[k] ** (1 / 3))))
+        pos = gsph.scene_block([0.06, 0.06, 0.06], side, side, side, 0.01, 0.01, seed + k)
+        w = 2 * side * 0.01
+        srv.submit([w, w, w], pos, int(steps[k]), float(arrivals[k]))
+    srv.run(realtime=True)
+    st = srv.stats()
+    srv.close()
+    return {
+        "workload": f"{requests} requests, Poisson rate {lam:.2f}/s (0.7 x batched capacity), "
+                    "N log-uniform 8192..1048576, steps 50..500, 200 GB HBM budget",
+        "completed": st["completed"], "failed": st["failed"], "rejected": st["rejected"],
+        "p50_latency_s": st["p50"], "p99_latency_s": st["p99"],
+        "particle_steps_per_s": st["particle_steps"] / st["wall"], "wall_s": st["wall"],
+        "max_resident": st["max_active"], "recompositions": st["recompositions"],
+        "engine_steps": st["engine_steps"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +162,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--legs", default="plain,batched,serving")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
@@ -139,6 +243,16 @@ def main():
         "checksum": [float(x) for x in cs],
     }
     s.close()
+    legs = args.legs.split(",")
+    if "batched" in legs:
+        print("[bench] batched leg", file=sys.stderr, flush=True)
+        out["batched"] = batched_leg(gsph, max(args.steps // 2, 5), 3)
+        print(f"[bench] batched {out['batched']['speedup_vs_plain_loop']:.2f}x plain loop",
+              file=sys.stderr, flush=True)
+    if "serving" in legs:
+        print("[bench] serving leg", file=sys.stderr, flush=True)
+        cap = out.get("batched", {}).get("particle_steps_per_s", value)
+        out["serving"] = serving_leg(gsph, cap)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         print(f"[bench] cpu baseline {args.cpu_steps} steps on {threads} threads",

@@ -18,6 +18,7 @@
 
 #include "gsph.h"
 #include "host_util.h"
+#include "engine.h"
 #include "kernels.h"
 
 using namespace gsph;
@@ -46,6 +47,7 @@ struct gsph_ctx {
     std::vector<SceneGeom> scenes;
     int n = 0, numc = 0, passes = 0;
     int sort_tiles = 0, cell_tiles = 0, nchunks = 0, split_tiles = 0, task_cap = 0;
+    int cap_n = 0, cap_c = 0, cap_s = 0;  // allocated capacity
     int grid_blocks = 0;
     int device = 0;
     hipStream_t stream = nullptr;
@@ -341,12 +343,106 @@ int bits_for(long long maxval) {
     return b;
 }
 
-int build_context(const gsph_params *p, const gsph_options *opt,
-                  const std::vector<SceneGeom> &scenes_in, int n, const float *pos4,
-                  const float *vel3, bool batch, gsph_ctx **out) {
+// Assigns cell bases to the scenes and derives every launch size from the
+// active particle / cell counts.  Fails if they exceed the allocated capacity.
+int configure_layout(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n) {
+    long long numc = 0, pb = 0;
+    for (auto &g : scenes) {
+        if (g.gx <= 0 || g.gy <= 0 || g.gz <= 0 || g.gx > 16383 || g.gy > 16383 || g.gz > 16383)
+            return fail(ctx, GSPH_ERR_INVALID_ARG, "grid dimensions must be in [1, 16383]");
+        g.cell_base = (int)numc;
+        g.p_begin = (int)pb;
+        numc += (long long)g.gx * g.gy * g.gz;
+        pb += g.p_count;
+        if (numc > (1LL << 26))
+            return fail(ctx, GSPH_ERR_CAPACITY,
+                        "more than 2^26 grid cells (micro ids must fit 32 bits)");
+    }
+    if (pb != n) return fail(ctx, GSPH_ERR_INVALID_ARG, "scene counts do not add up");
+    if (n > ctx->cap_n || numc > ctx->cap_c || (int)scenes.size() > ctx->cap_s)
+        return fail(ctx, GSPH_ERR_CAPACITY, "layout exceeds the context capacity");
+    ctx->scenes = scenes;
+    ctx->n = n;
+    ctx->numc = (int)numc;
+    ctx->passes = (bits_for(std::max(numc * 64 - 1, 1LL)) + 7) / 8;
+    ctx->sort_tiles = std::max(1, (n + kSortTile - 1) / kSortTile);
+    ctx->cell_tiles = std::max(1, (n + kCellTile - 1) / kCellTile);
+    ctx->nchunks = std::max(1, (ctx->numc + kFillChunk - 1) / kFillChunk);
+    ctx->split_tiles = std::max(1, (n + kSplitTile - 1) / kSplitTile);
+    ctx->task_cap = (n + 31) / 32 + std::min(ctx->numc, n) + 2;
+    if (!scenes.empty()) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->scenes_d, ctx->scenes.data(),
+                                    sizeof(SceneGeom) * ctx->scenes.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    // launch sizes changed: a captured step graph is stale
+    if (ctx->graph_exec) {
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
+    if (ctx->graph) {
+        (void)hipGraphDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    ctx->grid_valid = false;
+    return GSPH_OK;
+}
+
+// All device buffers, sized by capacity (particles, cells, scenes).
+int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
+    ctx->cap_n = cap_n;
+    ctx->cap_c = cap_c;
+    ctx->cap_s = std::max(cap_s, 1);
+    const size_t N = (size_t)cap_n, C = (size_t)cap_c;
+    const size_t sort_tiles = std::max<size_t>(1, (N + kSortTile - 1) / kSortTile);
+    const size_t cell_tiles = std::max<size_t>(1, (N + kCellTile - 1) / kCellTile);
+    const size_t split_tiles = std::max<size_t>(1, (N + kSplitTile - 1) / kSplitTile);
+    const size_t nchunks = std::max<size_t>(1, (C + kFillChunk - 1) / kFillChunk);
+    const size_t task_cap = (N + 31) / 32 + std::min(C, N) + 2;
+    int rc;
+#define ALLOC(ptr, cnt)                          \
+    do {                                         \
+        rc = dalloc(ctx, &(ptr), (size_t)(cnt)); \
+        if (rc) return rc;                       \
+    } while (0)
+    for (int k = 0; k < 2; ++k) {
+        ALLOC(ctx->pos[k], N);
+        ALLOC(ctx->vel[k], N);
+        ALLOC(ctx->ev[k], N);
+        ALLOC(ctx->keys[k], N);
+        ALLOC(ctx->vals[k], N);
+    }
+    ALLOC(ctx->acc, N);
+    ALLOC(ctx->tilecnt, 256 * sort_tiles);
+    ALLOC(ctx->digit_total, 256);
+    ALLOC(ctx->bcnt, cell_tiles);
+    ALLOC(ctx->split_sum, 3 * split_tiles);
+    ALLOC(ctx->cells, N + 1);
+    ALLOC(ctx->cstart, N + 1);
+    ALLOC(ctx->chunk_k0, nchunks + 1);
+    ALLOC(ctx->begin, C + 1);
+    ALLOC(ctx->count, C);
+    ALLOC(ctx->req, C);
+    ALLOC(ctx->tra_base, C);
+    ALLOC(ctx->sms_base, C);
+    ALLOC(ctx->xcum, C);
+    ALLOC(ctx->compact, N);
+    ALLOC(ctx->tasks, task_cap);
+    ALLOC(ctx->dc, kCntSlots);
+    ALLOC(ctx->scenes_d, ctx->cap_s);
+    ALLOC(ctx->sum_partial, 5 * 256);
+    ALLOC(ctx->sum_out, 5);
+#undef ALLOC
+    HIP_TRY(ctx, hipMemsetAsync(ctx->dc, 0, sizeof(int) * kCntSlots, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->tasks, 0, sizeof(gsph_task) * task_cap, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return GSPH_OK;
+}
+
+// An empty context: constants, stream, events; no buffers yet.
+int new_context(const gsph_params *p, const gsph_options *opt, bool batch, gsph_ctx **out) {
     if (!p || !out) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
-    if (n < 0) return fail(nullptr, GSPH_ERR_INVALID_ARG, "negative particle count");
-    if (n > 0 && !pos4) return fail(nullptr, GSPH_ERR_INVALID_ARG, "pos4 is null");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(nullptr, GSPH_ERR_NO_DEVICE, "There is no HIP device.");
@@ -362,112 +458,63 @@ int build_context(const gsph_params *p, const gsph_options *opt,
         return rc;
     }
     ctx->batch = batch;
-    ctx->scenes = scenes_in;
-    ctx->n = n;
-    long long numc = 0;
-    for (auto &g : ctx->scenes) {
-        if (g.gx <= 0 || g.gy <= 0 || g.gz <= 0 || g.gx > 16383 || g.gy > 16383 || g.gz > 16383) {
-            delete ctx;
-            return fail(nullptr, GSPH_ERR_INVALID_ARG, "grid dimensions must be in [1, 16383]");
-        }
-        g.cell_base = (int)numc;
-        numc += (long long)g.gx * g.gy * g.gz;
-        if (numc > (1LL << 26)) {
-            delete ctx;
-            return fail(nullptr, GSPH_ERR_CAPACITY,
-                        "more than 2^26 grid cells (micro ids must fit 32 bits)");
-        }
-    }
-    ctx->numc = (int)numc;
-    ctx->passes = (bits_for(numc * 64 - 1) + 7) / 8;
-    ctx->sort_tiles = std::max(1, (n + kSortTile - 1) / kSortTile);
-    ctx->cell_tiles = std::max(1, (n + kCellTile - 1) / kCellTile);
-    ctx->nchunks = std::max(1, (ctx->numc + kFillChunk - 1) / kFillChunk);
-    ctx->split_tiles = std::max(1, (n + kSplitTile - 1) / kSplitTile);
-    ctx->task_cap = (n + 31) / 32 + std::min(ctx->numc, n) + 2;
-
     hipDeviceProp_t prop;
     (void)hipGetDevice(&ctx->device);
     int cus = 256;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;
-    int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 8;
+    const int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 8;
     ctx->grid_blocks = std::max(8, (cus * bpc + 7) / 8 * 8);
-
-#define ALLOC(ptr, cnt)                         \
-    do {                                        \
-        rc = dalloc(ctx, &(ptr), (size_t)(cnt)); \
-        if (rc) goto failed;                    \
-    } while (0)
-    {
-        const size_t N = (size_t)n, C = (size_t)ctx->numc;
-        for (int k = 0; k < 2; ++k) {
-            ALLOC(ctx->pos[k], N);
-            ALLOC(ctx->vel[k], N);
-            ALLOC(ctx->ev[k], N);
-            ALLOC(ctx->keys[k], N);
-            ALLOC(ctx->vals[k], N);
-        }
-        ALLOC(ctx->acc, N);
-        ALLOC(ctx->tilecnt, 256 * (size_t)ctx->sort_tiles);
-        ALLOC(ctx->digit_total, 256);
-        ALLOC(ctx->bcnt, ctx->cell_tiles);
-        ALLOC(ctx->split_sum, 3 * (size_t)ctx->split_tiles);
-        ALLOC(ctx->cells, N + 1);
-        ALLOC(ctx->cstart, N + 1);
-        ALLOC(ctx->chunk_k0, ctx->nchunks + 1);
-        ALLOC(ctx->begin, C + 1);
-        ALLOC(ctx->count, C);
-        ALLOC(ctx->req, C);
-        ALLOC(ctx->tra_base, C);
-        ALLOC(ctx->sms_base, C);
-        ALLOC(ctx->xcum, C);
-        ALLOC(ctx->compact, N);
-        ALLOC(ctx->tasks, ctx->task_cap);
-        ALLOC(ctx->dc, kCntSlots);
-        ALLOC(ctx->scenes_d, ctx->scenes.size());
-        ALLOC(ctx->sum_partial, 5 * 256);
-        ALLOC(ctx->sum_out, 5);
-    }
-#undef ALLOC
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        rc = fail(nullptr, GSPH_ERR_HIP, "hipStreamCreate failed");
-        goto failed;
+        delete ctx;
+        return fail(nullptr, GSPH_ERR_HIP, "hipStreamCreate failed");
     }
     for (auto &e : ctx->stage_ev) (void)hipEventCreate(&e);
-    {
-        hipError_t e = hipMemcpyAsync(ctx->scenes_d, ctx->scenes.data(),
-                                      sizeof(SceneGeom) * ctx->scenes.size(),
+    *out = ctx;
+    return GSPH_OK;
+}
+
+int build_context(const gsph_params *p, const gsph_options *opt,
+                  const std::vector<SceneGeom> &scenes_in, int n, const float *pos4,
+                  const float *vel3, bool batch, gsph_ctx **out) {
+    if (!p || !out) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    if (n < 0) return fail(nullptr, GSPH_ERR_INVALID_ARG, "negative particle count");
+    if (n > 0 && !pos4) return fail(nullptr, GSPH_ERR_INVALID_ARG, "pos4 is null");
+    gsph_ctx *ctx = nullptr;
+    int rc = new_context(p, opt, batch, &ctx);
+    if (rc) return rc;
+    long long numc = 0;
+    for (auto &g : scenes_in) numc += (long long)std::max(g.gx, 0) * std::max(g.gy, 0) * std::max(g.gz, 0);
+    if (numc > (1LL << 26)) {
+        gsph_destroy(ctx);
+        return fail(nullptr, GSPH_ERR_CAPACITY, "more than 2^26 grid cells (micro ids must fit 32 bits)");
+    }
+    rc = alloc_buffers(ctx, n, (int)numc, (int)scenes_in.size());
+    if (!rc) rc = configure_layout(ctx, scenes_in, n);
+    if (!rc && n > 0) {
+        hipError_t e = hipMemcpyAsync(ctx->pos[0], pos4, sizeof(float4) * (size_t)n,
                                       hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(ctx->dc, 0, sizeof(int) * kCntSlots, ctx->stream);
-        if (e == hipSuccess) e = hipMemsetAsync(ctx->tasks, 0, sizeof(gsph_task) * ctx->task_cap, ctx->stream);
-        if (e == hipSuccess && n > 0)
-            e = hipMemcpyAsync(ctx->pos[0], pos4, sizeof(float4) * (size_t)n,
-                               hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess && n > 0) {
+        if (e == hipSuccess) {
             if (vel3)
                 e = hipMemcpyAsync(ctx->vel[0], vel3, sizeof(f3) * (size_t)n,
                                    hipMemcpyHostToDevice, ctx->stream);
             else
                 e = hipMemsetAsync(ctx->vel[0], 0, sizeof(f3) * (size_t)n, ctx->stream);
         }
-        if (e != hipSuccess) {
-            rc = fail(nullptr, GSPH_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
-            goto failed;
+        if (e == hipSuccess) {
+            launch_init_state(ctx->ev[0], ctx->acc, n, ctx->stream);
+            e = hipStreamSynchronize(ctx->stream);
+            if (e == hipSuccess) e = hipGetLastError();
         }
-        launch_init_state(ctx->ev[0], ctx->acc, n, ctx->stream);
-        e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess) e = hipGetLastError();
-        if (e != hipSuccess) {
-            rc = fail(nullptr, GSPH_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
-            goto failed;
-        }
+        if (e != hipSuccess) rc = fail(ctx, GSPH_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        set_error(nullptr, ctx->err);
+        gsph_destroy(ctx);
+        return rc;
     }
     ctx->cur = 0;
     *out = ctx;
     return GSPH_OK;
-failed:
-    gsph_destroy(ctx);
-    return rc;
 }
 
 SceneGeom geom_for(const gsph_constants &c, const float world[3], int p_begin, int p_count) {
@@ -512,6 +559,120 @@ int capture_graph(gsph_ctx *ctx) {
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Engine hooks of the serving loop (engine.h).
+namespace gsph {
+
+size_t engine_bytes(long long n, long long cells) {
+    // per particle: 2 x state (44 B), acc, 2 x key/value, cell list, compact,
+    // task slot; per cell: begin, count, req, TRA/SMS bases, x-layer starts
+    return (size_t)(n * 196 + cells * 28 + (1 << 20));
+}
+
+int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,
+                  int cap_s, gsph_ctx **out) {
+    gsph_ctx *ctx = nullptr;
+    int rc = new_context(p, opt, true, &ctx);
+    if (rc) return rc;
+    rc = alloc_buffers(ctx, cap_n, cap_c, cap_s);
+    if (!rc) rc = configure_layout(ctx, {}, 0);
+    if (rc) {
+        set_error(nullptr, ctx->err);
+        gsph_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return GSPH_OK;
+}
+
+void engine_capacity(const gsph_ctx *ctx, int *cap_n, int *cap_c, int *cap_s) {
+    *cap_n = ctx->cap_n;
+    *cap_c = ctx->cap_c;
+    *cap_s = ctx->cap_s;
+}
+
+SceneGeom engine_geom(const gsph_ctx *ctx, const float world[3], int count) {
+    return geom_for(ctx->c, world, 0, count);
+}
+
+// New scene set.  Scene k keeps its state from slot range [old_begin, +count)
+// of the current resident buffer, or (old_begin < 0) is uploaded from host
+// arrays with ev = 0.  The new layout is staged in the sorted buffer and then
+// becomes the resident one.  `dst` may be a different, larger context: the
+// state is then copied across and the caller replaces the old context.
+int engine_recompose(gsph_ctx *src, gsph_ctx *dst, const std::vector<EngineScene> &scenes) {
+    std::vector<SceneGeom> geo;
+    long long n = 0;
+    for (const auto &e : scenes) {
+        geo.push_back(geom_for(dst->c, e.world, (int)n, e.count));
+        n += e.count;
+    }
+    if (n > dst->cap_n) return fail(dst, GSPH_ERR_CAPACITY, "engine particle capacity");
+    if (src && src->cur != 0) return fail(dst, GSPH_ERR_INVALID_ARG, "recompose needs integrated state");
+    hipStream_t st = dst->stream;
+    if (src && src != dst) HIP_TRY(dst, hipStreamSynchronize(src->stream));
+    long long off = 0;
+    for (const auto &e : scenes) {
+        const size_t c = (size_t)e.count;
+        if (c == 0) continue;
+        if (e.old_begin >= 0) {
+            HIP_TRY(dst, hipMemcpyAsync(dst->pos[1] + off, src->pos[0] + e.old_begin, 16 * c,
+                                        hipMemcpyDeviceToDevice, st));
+            HIP_TRY(dst, hipMemcpyAsync(dst->vel[1] + off, src->vel[0] + e.old_begin, 12 * c,
+                                        hipMemcpyDeviceToDevice, st));
+            HIP_TRY(dst, hipMemcpyAsync(dst->ev[1] + off, src->ev[0] + e.old_begin, 16 * c,
+                                        hipMemcpyDeviceToDevice, st));
+        } else {
+            HIP_TRY(dst, hipMemcpyAsync(dst->pos[1] + off, e.pos4, 16 * c,
+                                        hipMemcpyHostToDevice, st));
+            if (e.vel3)
+                HIP_TRY(dst, hipMemcpyAsync(dst->vel[1] + off, e.vel3, 12 * c,
+                                            hipMemcpyHostToDevice, st));
+            else
+                HIP_TRY(dst, hipMemsetAsync(dst->vel[1] + off, 0, 12 * c, st));
+            HIP_TRY(dst, hipMemsetAsync(dst->ev[1] + off, 0, 16 * c, st));
+        }
+        off += (long long)c;
+    }
+    HIP_TRY(dst, hipStreamSynchronize(st));
+    std::swap(dst->pos[0], dst->pos[1]);
+    std::swap(dst->vel[0], dst->vel[1]);
+    std::swap(dst->ev[0], dst->ev[1]);
+    int rc = configure_layout(dst, geo, (int)n);
+    dst->cur = 0;
+    return rc;
+}
+
+int engine_read_scene(gsph_ctx *ctx, int s, float *pos4, float *vel3, float *ev4) {
+    if (s < 0 || s >= (int)ctx->scenes.size()) return fail(ctx, GSPH_ERR_INVALID_ARG, "scene index");
+    const SceneGeom &g = ctx->scenes[s];
+    const size_t c = (size_t)g.p_count;
+    const int b = ctx->cur;
+    if (c) {
+        if (pos4) HIP_TRY(ctx, hipMemcpyAsync(pos4, ctx->pos[b] + g.p_begin, 16 * c, hipMemcpyDeviceToHost, ctx->stream));
+        if (vel3) HIP_TRY(ctx, hipMemcpyAsync(vel3, ctx->vel[b] + g.p_begin, 12 * c, hipMemcpyDeviceToHost, ctx->stream));
+        if (ev4) HIP_TRY(ctx, hipMemcpyAsync(ev4, ctx->ev[b] + g.p_begin, 16 * c, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return GSPH_OK;
+}
+
+// Clears the sticky device error flags; returns them.
+int engine_take_errors(gsph_ctx *ctx, int *flags) {
+    int f = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&f, ctx->dc + kCntErr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (f) {
+        const int zero = 0;
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->dc + kCntErr, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    *flags = f;
+    return GSPH_OK;
+}
+
+}  // namespace gsph
 
 // ---------------------------------------------------------------------------
 extern "C" {

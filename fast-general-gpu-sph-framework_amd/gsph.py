@@ -87,6 +87,16 @@ class PairStats(C.Structure):
         "partial_is_same_pairs", "compact_is_same_pairs", "widened_is_same_pairs")]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int), ("failed", C.c_int), ("rejected", C.c_int),
+                ("max_active", C.c_int), ("p50", C.c_double), ("p99", C.c_double),
+                ("wall", C.c_double), ("particle_steps", C.c_double),
+                ("engine_steps", C.c_longlong), ("recompositions", C.c_longlong),
+                ("peak_bytes", C.c_double)]
+
+
+REQ_STATES = {0: "queued", 1: "active", 2: "done", 3: "failed", 4: "rejected"}
+
 # 32-byte task record (gsph_task / the reference's BlockTask)
 TASK_DTYPE = np.dtype([
     ("is_same", "i1"), ("pad0", "u1"), ("paired_bounds", "<u2"), ("cellid", "<i4"),
@@ -133,6 +143,15 @@ _SIGS = {
     "gsph_kernel_timing": (_I, [_P, _I]),
     "gsph_kernel_times": (_I, [_P, C.c_char_p, _I, _P, _P, _I, C.POINTER(_I)]),
     "gsph_inject_fault": (_I, [_P, _I]),
+    "gsph_server_create": (_I, [C.POINTER(Params), C.POINTER(Options), C.c_double, _I,
+                                C.POINTER(_P)]),
+    "gsph_server_destroy": (_I, [_P]),
+    "gsph_server_submit": (_I, [_P, _P, _I, _P, _P, _I, C.c_double, C.POINTER(_I)]),
+    "gsph_server_run": (_I, [_P, _I]),
+    "gsph_server_result": (_I, [_P, _I, _P, _P, _P, C.POINTER(_I), C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)]),
+    "gsph_server_get_stats": (_I, [_P, C.POINTER(ServerStats)]),
+    "gsph_server_last_error": (C.c_char_p, [_P]),
 }
 
 # Every symbol include/gsph.h declares (checked by the CPU tests).
@@ -361,3 +380,66 @@ class System:
 
     def inject_fault(self, after_calls=0):
         self._c(self._lib.gsph_inject_fault(self.h, after_calls))
+
+
+class Server:
+    """Continuous-batching request loop (gsph_server_*)."""
+
+    def __init__(self, params: Params, hbm_budget_bytes: float, max_scenes: int = 1024,
+                 options: Options | None = None):
+        lib = load()
+        self._lib = lib
+        opt = options if options is not None else options_default()
+        h = C.c_void_p()
+        _check(lib.gsph_server_create(C.byref(params), C.byref(opt), float(hbm_budget_bytes),
+                                      int(max_scenes), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def _c(self, rc):
+        if rc != 0:
+            msg = self._lib.gsph_server_last_error(self.h)
+            raise GsphError(rc, msg.decode() if msg else "")
+
+    def submit(self, world, pos4, steps, arrival=0.0, vel3=None) -> int:
+        w = np.ascontiguousarray(world, np.float32)
+        pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        vel = None if vel3 is None else np.ascontiguousarray(vel3, np.float32)
+        rid = C.c_int()
+        self._c(self._lib.gsph_server_submit(self.h, _ptr(w), pos4.shape[0], _ptr(pos4),
+                                             _ptr(vel), int(steps), float(arrival),
+                                             C.byref(rid)))
+        self.sizes[rid.value] = pos4.shape[0]
+        return rid.value
+
+    def run(self, realtime=False):
+        self._c(self._lib.gsph_server_run(self.h, 1 if realtime else 0))
+
+    def result(self, rid):
+        n = self.sizes[rid]
+        pos = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        ev = np.zeros((n, 4), np.float32)
+        st = C.c_int()
+        lat = C.c_double()
+        q = C.c_double()
+        self._c(self._lib.gsph_server_result(self.h, rid, _ptr(pos), _ptr(vel), _ptr(ev),
+                                             C.byref(st), C.byref(lat), C.byref(q)))
+        return dict(state=REQ_STATES[st.value], latency=lat.value, queued=q.value,
+                    pos=pos, vel=vel, ev=ev)
+
+    def stats(self):
+        st = ServerStats()
+        self._c(self._lib.gsph_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            self._lib.gsph_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

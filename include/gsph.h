@@ -170,6 +170,41 @@ int gsph_batch_create(const gsph_params *p, const gsph_options *opt, int nscenes
 int gsph_scene_count(const gsph_ctx *ctx, int *nscenes);
 int gsph_scene_range(const gsph_ctx *ctx, int scene, int *first, int *count);
 
+/* Serving form --------------------------------------------------------------- */
+/* No counterpart in the reference.  A continuous-batching request loop:
+ * requests (one scene + a step count) are admitted in arrival order against
+ * an HBM budget and a scene-slot limit, all resident scenes advance together
+ * one step per engine iteration, finished scenes are copied out and their
+ * memory reused.  A request larger than the budget, or with particles outside
+ * its box, is rejected at submission; a scene whose state stops being finite
+ * fails alone.  run(realtime = 1) takes arrival times in seconds against the
+ * wall clock; realtime = 0 counts them in engine iterations. */
+typedef struct gsph_server gsph_server;
+typedef struct gsph_server_stats {
+    int completed, failed, rejected, max_active;
+    double p50, p99;          /* request latency (arrival -> result), s or iterations */
+    double wall;              /* s */
+    double particle_steps;
+    long long engine_steps, recompositions;
+    double peak_bytes;        /* admitted HBM estimate */
+} gsph_server_stats;
+#define GSPH_REQ_QUEUED 0
+#define GSPH_REQ_ACTIVE 1
+#define GSPH_REQ_DONE 2
+#define GSPH_REQ_FAILED 3
+#define GSPH_REQ_REJECTED 4
+int gsph_server_create(const gsph_params *p, const gsph_options *opt,
+                       double hbm_budget_bytes, int max_scenes, gsph_server **out);
+int gsph_server_destroy(gsph_server *s);
+int gsph_server_submit(gsph_server *s, const float world3[3], int n,
+                       const float *pos4, const float *vel3, int steps,
+                       double arrival, int *id);
+int gsph_server_run(gsph_server *s, int realtime);
+int gsph_server_result(gsph_server *s, int id, float *pos4, float *vel3,
+                       float *ev4, int *state, double *latency, double *queued);
+int gsph_server_get_stats(gsph_server *s, gsph_server_stats *out);
+const char *gsph_server_last_error(const gsph_server *s);
+
 /* Plain single scene -------------------------------------------------------- */
 /* HybridSystem ctor + initializeScene + BuffInit (sph_hybrid_system.cpp:209-240,
  * 432-473; kernel_dispatch.cu:17-22): uploads n particles, ev = acc = 0. */
