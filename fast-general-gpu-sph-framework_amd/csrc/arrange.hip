@@ -134,12 +134,17 @@ __global__ __launch_bounds__(kNT) void k_rowscan(ArrangeArgs a) {
 }
 
 // Stable scatter of one radix pass.  dest = (keys with a smaller digit) +
-// (same digit in earlier tiles) + (same digit earlier in this tile).
+// (same digit in earlier tiles) + (same digit earlier in this tile).  The tile
+// is first reordered by digit in LDS, then written out run by run so that
+// consecutive threads store to consecutive addresses.
 template <bool kFirst>
 __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
     __shared__ uint32_t s_w[4][256];
-    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_base[256];      // global destination of the tile's digit run
+    __shared__ uint32_t s_local[256];     // start of the digit run inside the tile
     __shared__ uint32_t s_tmp[8];
+    __shared__ uint32_t s_key[kSortTile];
+    __shared__ int s_val[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = blockIdx.x;
     const int shift = 8 * pass;
@@ -155,6 +160,7 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
     }
     __syncthreads();
     const int base = tile * kSortTile + wave * 1024;
+    const int tile_n = min(kSortTile, a.n - tile * kSortTile);
     uint32_t key[kItems];
     int val[kItems];
     uint32_t rank[kItems];
@@ -182,6 +188,7 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
         if (valid && lt == 0) s_w[wave][d] = old + (uint32_t)__popcll(peers);
     }
     __syncthreads();
+    uint32_t cnt_d;
     {
         uint32_t run = 0;
 #pragma unroll
@@ -190,6 +197,11 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
             s_w[w][tid] = run;
             run += c;
         }
+        cnt_d = run;
+    }
+    {
+        uint32_t tot;
+        s_local[tid] = block_excl_scan<kNT>(cnt_d, s_tmp, tot);
     }
     __syncthreads();
 #pragma unroll
@@ -197,10 +209,18 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
         const int i = base + s * 64 + lane;
         if (i < a.n) {
             const uint32_t d = (key[s] >> shift) & 255u;
-            const uint32_t dest = s_base[d] + s_w[wave][d] + rank[s];
-            kout[dest] = key[s];
-            vout[dest] = val[s];
+            const uint32_t lp = s_local[d] + s_w[wave][d] + rank[s];
+            s_key[lp] = key[s];
+            s_val[lp] = val[s];
         }
+    }
+    __syncthreads();
+    for (int k = tid; k < tile_n; k += kNT) {
+        const uint32_t kk = s_key[k];
+        const uint32_t d = (kk >> shift) & 255u;
+        const uint32_t dest = s_base[d] + ((uint32_t)k - s_local[d]);
+        kout[dest] = kk;
+        vout[dest] = s_val[k];
     }
 }
 
@@ -226,10 +246,14 @@ __global__ __launch_bounds__(kNT) void k_gather(ArrangeArgs a) {
 }
 
 // Occupied-cell list.  The tile prefix is re-derived by every block from the
-// per-tile boundary counts (a few thousand at most).
+// per-tile boundary counts (a few thousand at most).  Items are wave-striped
+// (slot s, thread t -> particle 4096 tile + 256 s + t, coalesced) and ranked
+// by ballots: the (slot, wave, lane) order is the particle order.
 __global__ __launch_bounds__(kNT) void k_cells(ArrangeArgs a) {
     __shared__ uint32_t s_tmp[8];
-    const int tid = threadIdx.x;
+    __shared__ uint32_t s_base[kCellTile / kNT][kNT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int kSlots = kCellTile / kNT;
     uint32_t pre = 0;
     for (int t = tid; t < (int)blockIdx.x; t += kNT) pre += a.bcnt[t];
     {
@@ -237,34 +261,51 @@ __global__ __launch_bounds__(kNT) void k_cells(ArrangeArgs a) {
         block_excl_scan<kNT>(pre, s_tmp, tot);
         pre = tot;
     }
-    // each thread owns 16 consecutive sorted particles
-    const int i0 = blockIdx.x * kCellTile + tid * (kCellTile / kNT);
-    uint32_t mine = 0;
-    for (int s = 0; s < kCellTile / kNT; ++s) {
-        const int i = i0 + s;
-        if (i >= a.n) break;
-        mine += (i == 0 || (a.hash[i - 1] >> 6) != (a.hash[i] >> 6)) ? 1u : 0u;
+    const int t0 = blockIdx.x * kCellTile;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int i = t0 + s * kNT + tid;
+        bool b = false;
+        if (i < a.n) b = i == 0 || (a.hash[i - 1] >> 6) != (a.hash[i] >> 6);
+        bits |= (b ? 1u : 0u) << s;
+        const uint64_t m = __ballot(b);
+        if (lane == 0) s_base[s][wave] = (uint32_t)__popcll(m);
     }
-    uint32_t tot;
-    uint32_t k = pre + block_excl_scan<kNT>(mine, s_tmp, tot);
-    for (int s = 0; s < kCellTile / kNT; ++s) {
-        const int i = i0 + s;
-        if (i >= a.n) break;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t run = pre;
+        for (int s = 0; s < kSlots; ++s)
+            for (int w = 0; w < kNT / 64; ++w) {
+                const uint32_t c = s_base[s][w];
+                s_base[s][w] = run;
+                run += c;
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int i = t0 + s * kNT + tid;
+        const bool b = (bits >> s) & 1u;
+        const uint64_t m = __ballot(b);
+        if (i >= a.n) continue;
         const int cell = (int)(a.hash[i] >> 6);
-        const int prev = i == 0 ? -1 : (int)(a.hash[i - 1] >> 6);
-        if (cell != prev) {
+        if (b) {
+            const int k = (int)(s_base[s][wave] + (uint32_t)__popcll(m & lanemask_lt()));
+            const int prev = i == 0 ? -1 : (int)(a.hash[i - 1] >> 6);
             a.cells[k] = cell;
             a.cstart[k] = i;
             // chunks whose first cell lies in (prev, cell] start at k
             const int q0 = prev < 0 ? 0 : prev / kFillChunk + 1;
             const int q1 = cell / kFillChunk;
-            for (int q = q0; q <= q1; ++q) a.chunk_k0[q] = (int)k;
-            ++k;
+            for (int q = q0; q <= q1; ++q) a.chunk_k0[q] = k;
         }
         if (i == a.n - 1) {
-            a.cstart[k] = a.n;
-            a.dc[kCntK] = (int)k;
-            for (int q = cell / kFillChunk + 1; q < a.nchunks; ++q) a.chunk_k0[q] = (int)k;
+            // the last particle closes the list (it sees every boundary before it)
+            const int K = (int)(s_base[s][wave] + (uint32_t)__popcll(m & lanemask_lt())) + (b ? 1 : 0);
+            a.cstart[K] = a.n;
+            a.dc[kCntK] = K;
+            for (int q = cell / kFillChunk + 1; q < a.nchunks; ++q) a.chunk_k0[q] = K;
         }
     }
 }
@@ -284,56 +325,81 @@ __device__ __forceinline__ int layer_bound(const uint32_t *hash, int lo, int hi,
 }
 
 // Dense begin[numc+1] / count[numc] plus x-layer starts of occupied cells.
-// Block q fills cells [4096 q, 4096 q + 4096) in 16 rounds of 256; the
-// occupied cells of a round are marked in a 256-bit LDS map.
+// Block q fills cells [4096 q, 4096 q + 4096) in 4 rounds of 1024; each
+// thread owns 4 consecutive cells (16-byte stores).  The occupied cells of a
+// round are marked in a 1024-bit LDS map; a thread's cursor into the
+// occupied-cell list is the map's population below its cells.
 __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
-    __shared__ uint32_t s_bits[8];
+    constexpr int kRound = 4 * kNT;
+    __shared__ uint32_t s_bits[kRound / 32];
+    __shared__ uint32_t s_pre[kRound / 32 + 1];
     const int tid = threadIdx.x;
     const int q = blockIdx.x;
     const int K = a.dc[kCntK];
     int kcur = a.chunk_k0[q];
-    for (int it = 0; it < kFillChunk / kNT; ++it) {
-        const int base = q * kFillChunk + it * kNT;
+    for (int it = 0; it < kFillChunk / kRound; ++it) {
+        const int base = q * kFillChunk + it * kRound;
         if (base >= a.numc) break;
-        if (tid < 8) s_bits[tid] = 0;
+        if (tid < kRound / 32) s_bits[tid] = 0;
         __syncthreads();
-        const int kk = kcur + tid;
-        if (kk < K) {
-            const int cid = a.cells[kk];
-            if (cid < base + kNT) {
-                const int off = cid - base;
-                atomicOr(&s_bits[off >> 5], 1u << (off & 31));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int kk = kcur + tid + kNT * j;
+            if (kk < K) {
+                const int cid = a.cells[kk];
+                if (cid < base + kRound) {
+                    const int off = cid - base;
+                    atomicOr(&s_bits[off >> 5], 1u << (off & 31));
+                }
             }
         }
         __syncthreads();
-        const int w = tid >> 5, b = tid & 31;
-        int below = __popc(s_bits[w] & ((1u << b) - 1u));
-        int total = 0;
+        if (tid < 64) {   // prefix of the 32 word populations (first wave)
+            const uint32_t pc = tid < kRound / 32 ? (uint32_t)__popc(s_bits[tid]) : 0u;
+            uint32_t x = pc;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int pc = __popc(s_bits[j]);
-            total += pc;
-            if (j < w) below += pc;
+            for (int o = 1; o < 32; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (tid >= o) x += y;
+            }
+            if (tid < kRound / 32) s_pre[tid] = x - pc;
+            if (tid == kRound / 32 - 1) s_pre[kRound / 32] = x;
         }
-        const bool occ = (s_bits[w] >> b) & 1u;
-        const int k = kcur + below;
-        const int c = base + tid;
-        if (c < a.numc) {
-            const int bg = k < K ? a.cstart[k] : a.n;
-            a.begin[c] = bg;
-            const int cnt = occ ? a.cstart[k + 1] - bg : 0;
-            a.count[c] = cnt;
-            if (occ) {
+        __syncthreads();
+        const int w = tid >> 3, sh = (tid & 7) * 4;
+        const uint32_t word = s_bits[w];
+        const uint32_t nib = (word >> sh) & 15u;
+        int k = kcur + (int)s_pre[w] + __popc(word & ((1u << sh) - 1u));
+        const int c0 = base + 4 * tid;
+        int bg[4], cn[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool occ = (nib >> u) & 1u;
+            bg[u] = k < K ? a.cstart[k] : a.n;
+            cn[u] = occ ? a.cstart[k + 1] - bg[u] : 0;
+            if (occ && c0 + u < a.numc) {
+                const int cnt = cn[u];
                 if (cnt > 65535) atomicOr(&a.dc[kCntErr], kErrCapacity);
-                const int e = bg + cnt;
-                const int l1 = layer_bound(a.hash, bg, e, 1);
+                const int e = bg[u] + cnt;
+                const int l1 = layer_bound(a.hash, bg[u], e, 1);
                 const int l2 = layer_bound(a.hash, l1, e, 2);
                 const int l3 = layer_bound(a.hash, l2, e, 3);
-                a.xcum[c] = make_uint2((uint32_t)(l1 - bg) | ((uint32_t)(l2 - bg) << 16),
-                                       (uint32_t)(l3 - bg));
+                a.xcum[c0 + u] = make_uint2((uint32_t)(l1 - bg[u]) | ((uint32_t)(l2 - bg[u]) << 16),
+                                            (uint32_t)(l3 - bg[u]));
             }
+            k += occ ? 1 : 0;
         }
-        kcur += total;
+        if (c0 + 3 < a.numc) {
+            *reinterpret_cast<int4 *>(a.begin + c0) = make_int4(bg[0], bg[1], bg[2], bg[3]);
+            *reinterpret_cast<int4 *>(a.count + c0) = make_int4(cn[0], cn[1], cn[2], cn[3]);
+        } else {
+            for (int u = 0; u < 4; ++u)
+                if (c0 + u < a.numc) {
+                    a.begin[c0 + u] = bg[u];
+                    a.count[c0 + u] = cn[u];
+                }
+        }
+        kcur += (int)s_pre[kRound / 32];
         __syncthreads();
     }
     if (q == a.nchunks - 1 && tid == 0) a.begin[a.numc] = a.n;

@@ -644,6 +644,10 @@ int engine_recompose(gsph_ctx *src, gsph_ctx *dst, const std::vector<EngineScene
     return rc;
 }
 
+int engine_scene_begin(const gsph_ctx *ctx, int s) {
+    return (s >= 0 && s < (int)ctx->scenes.size()) ? ctx->scenes[s].p_begin : -1;
+}
+
 int engine_read_scene(gsph_ctx *ctx, int s, float *pos4, float *vel3, float *ev4) {
     if (s < 0 || s >= (int)ctx->scenes.size()) return fail(ctx, GSPH_ERR_INVALID_ARG, "scene index");
     const SceneGeom &g = ctx->scenes[s];
@@ -1022,6 +1026,36 @@ int gsph_kernel_times(gsph_ctx *ctx, char *names, int names_cap, float *ms, int 
         names[names_cap - 1] = 0;
     }
     return GSPH_OK;
+}
+
+int gsph_probe_copy(double bytes, int reps, float *ms) {
+    if (bytes <= 0 || reps <= 0) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe size");
+    const size_t n = (size_t)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GSPH_OK;
+    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(nullptr, GSPH_ERR_HIP, "probe allocation failed");
+    } else {
+        (void)hipMemsetAsync(a, 0, n * 16, s);
+        launch_copy(a, b, n, s);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) launch_copy(a, b, n, s);
+        (void)hipEventRecord(e1, s);
+        if (hipStreamSynchronize(s) != hipSuccess) rc = fail(nullptr, GSPH_ERR_HIP, "probe failed");
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (ms) *ms = t / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
 }
 
 int gsph_inject_fault(gsph_ctx *ctx, int after_calls) {

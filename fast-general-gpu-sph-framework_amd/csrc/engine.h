@@ -12,7 +12,7 @@ namespace gsph {
 struct EngineScene {
     float world[3];
     int count;
-    int old_begin;          // slot of the scene in the current layout, or -1 (new)
+    int old_begin;          // first state slot of the scene in the current layout, or -1 (new)
     const float *pos4;      // new scenes: host state
     const float *vel3;      // may be null (zero)
 };
@@ -24,6 +24,8 @@ void engine_capacity(const gsph_ctx *ctx, int *cap_n, int *cap_c, int *cap_s);
 SceneGeom engine_geom(const gsph_ctx *ctx, const float world[3], int count);
 int engine_recompose(gsph_ctx *src, gsph_ctx *dst, const std::vector<EngineScene> &scenes);
 int engine_read_scene(gsph_ctx *ctx, int s, float *pos4, float *vel3, float *ev4);
+// First state slot of scene s in the current layout.
+int engine_scene_begin(const gsph_ctx *ctx, int s);
 int engine_take_errors(gsph_ctx *ctx, int *flags);
 
 }  // namespace gsph

@@ -109,5 +109,6 @@ void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_checksum(const float4 *pos, int n, double *partial, int nblocks,
                      double *out5, hipStream_t s);
 void launch_init_state(float4 *ev, f3 *acc, int n, hipStream_t s);
+void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s);
 
 }  // namespace gsph

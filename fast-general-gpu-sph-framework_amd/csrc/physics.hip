@@ -22,6 +22,8 @@
 // among that XCD's waves.
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace gsph {
 namespace {
 
@@ -191,7 +193,21 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 int lo, hi;
                 row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r],
                           rp.gr[r], lo, hi);
-                for (int j = lo; j < hi; ++j) s += density_term(pi, P.pos[j], h2);
+                // per-row partial sum (halves the rounding of ~100-term sums);
+                // rejected candidates add exact zeros, so pruning cannot
+                // change the result
+                float rs = 0.f;
+                int j = lo;
+                for (; j + 4 <= hi; j += 4) {
+                    const float4 q0 = P.pos[j], q1 = P.pos[j + 1];
+                    const float4 q2 = P.pos[j + 2], q3 = P.pos[j + 3];
+                    rs += density_term(pi, q0, h2);
+                    rs += density_term(pi, q1, h2);
+                    rs += density_term(pi, q2, h2);
+                    rs += density_term(pi, q3, h2);
+                }
+                for (; j < hi; ++j) rs += density_term(pi, P.pos[j], h2);
+                s += rs;
             }
             density_store(P, i, s);
         } else {
@@ -220,13 +236,15 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 if (has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
                     row_range(g, P.begin, P.xcum, cx, ny, nz, max(gl, 0),
                               min(gr, g.gx * 4 - 1), lo, hi);
+                float rs = 0.f;
                 for (int b0 = lo; b0 < hi; b0 += 32) {
                     const int cnt = min(32, hi - b0);
                     if (hl < cnt) lds[hl] = P.pos[b0 + hl];
                     wave_sync_lds();
-                    for (int k = 0; k < cnt; ++k) s += density_term(pi, lds[k], h2);
+                    for (int k = 0; k < cnt; ++k) rs += density_term(pi, lds[k], h2);
                     wave_sync_lds();
                 }
+                s += rs;
             }
             if (active) density_store(P, self, s);
         }
@@ -239,16 +257,45 @@ struct ForceAcc {
     float px, py, pz, vx, vy, vz, gx, gy, gz, l;
 };
 
-__device__ __forceinline__ void force_term(const float4 &pi, const float4 &ei,
-                                           const float4 &pj, const float4 &ej,
-                                           const gsph_constants &c, ForceAcc &A) {
+// h^2 - r^2 to ~1 ulp of the result.  Near r = h the plain FP32 difference
+// cancels (its error is ~1 ulp of h^2, i.e. relative h^2/(h^2 - r^2)), which
+// makes the kr^2 and (h^2 - r^2)^2 terms of a pair near the kernel edge
+// inaccurate.  The squares are split exactly (r_x^2 = a + ea by FMA), their
+// sum carried as a double-float (TwoSum), and h^2 = fl(h^2) + k2lo exactly.
+// Coordinates differences are exact (Sterbenz) for neighbours.  FP32 only.
+__device__ __forceinline__ float h2_minus_r2(float rx, float ry, float rz, float k2,
+                                             float k2lo) {
+    const float a = __fmul_rn(rx, rx), b = __fmul_rn(ry, ry), c = __fmul_rn(rz, rz);
+    const float ea = fmaf(rx, rx, -a), eb = fmaf(ry, ry, -b), ec = fmaf(rz, rz, -c);
+    const float s1 = __fadd_rn(a, b);
+    const float z1 = __fsub_rn(s1, a);
+    const float e1 = __fadd_rn(__fsub_rn(a, __fsub_rn(s1, z1)), __fsub_rn(b, z1));
+    const float s = __fadd_rn(s1, c);
+    const float z2 = __fsub_rn(s, s1);
+    const float e2 = __fadd_rn(__fsub_rn(s1, __fsub_rn(s, z2)), __fsub_rn(c, z2));
+    const float lo = __fadd_rn(__fadd_rn(ea, eb), __fadd_rn(ec, __fadd_rn(e1, e2)));
+    return __fadd_rn(__fsub_rn(k2, s), __fsub_rn(k2lo, lo));
+}
+
+// The reference's acceptance test of a pair: 1e-12 <= r^2 <= h^2 in FP32.
+__device__ __forceinline__ bool pair_accepted(const float4 &pi, const float4 &pj, float k2) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float d2 = rx * rx + ry * ry + rz * rz;
-    if (d2 < kFloatSmall || d2 > c.kernel_2) return;
+    return d2 >= kFloatSmall && d2 <= k2;
+}
+
+// Pressure, viscosity and colour-field terms of one accepted pair.
+__device__ __forceinline__ void force_pair(const float4 &pi, const float4 &ei,
+                                           const float4 &pj, const float4 &ej,
+                                           const gsph_constants &c, float k2lo,
+                                           ForceAcc &A) {
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float d2 = rx * rx + ry * ry + rz * rz;
     const float inv = __frsqrt_rn(d2);
     const float dist = d2 * inv;
     const float V = pj.w;
-    const float kr = c.kernel_h - dist;
+    const float t = h2_minus_r2(rx, ry, rz, c.kernel_2, k2lo);
+    const float kr = t / (c.kernel_h + dist);           // h - r without cancellation
     const float Vkr = V * kr;
     const float pw = Vkr * (ei.w + ej.w) * kr * inv;
     A.px -= rx * pw;
@@ -257,7 +304,6 @@ __device__ __forceinline__ void force_term(const float4 &pi, const float4 &ei,
     A.vx += (ej.x - ei.x) * Vkr;
     A.vy += (ej.y - ei.y) * Vkr;
     A.vz += (ej.z - ei.z) * Vkr;
-    const float t = c.kernel_2 - d2;
     const float Vt = V * t;
     const float gw = Vt * t;
     A.gx += rx * gw;
@@ -301,6 +347,8 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     const int tra_tiles = (tra + 63) >> 6;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
+    const float k2lo = fmaf(P.c.kernel_h, P.c.kernel_h, -P.c.kernel_2);  // h^2 - fl(h^2)
+    const float k2 = P.c.kernel_2;
     for (int it = m.first; it < m.end; it += m.stride) {
         if (it < tra_tiles) {
             const int idx = it * 64 + lane;
@@ -320,7 +368,28 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 int lo, hi;
                 row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r],
                           rp.gr[r], lo, hi);
-                for (int j = lo; j < hi; ++j) force_term(pi, ei, P.pos[j], P.ev[j], P.c, A);
+                // phase 1 tests up to 32 candidates into a mask, phase 2 runs the
+                // pair body over the accepted ones only (in candidate order)
+                for (int b0 = lo; b0 < hi; b0 += 32) {
+                    const int cnt = min(32, hi - b0);
+                    uint32_t mask = 0;
+                    int k = 0;
+                    for (; k + 4 <= cnt; k += 4) {
+                        const float4 q0 = P.pos[b0 + k], q1 = P.pos[b0 + k + 1];
+                        const float4 q2 = P.pos[b0 + k + 2], q3 = P.pos[b0 + k + 3];
+                        mask |= (pair_accepted(pi, q0, k2) ? 1u : 0u) << k;
+                        mask |= (pair_accepted(pi, q1, k2) ? 2u : 0u) << k;
+                        mask |= (pair_accepted(pi, q2, k2) ? 4u : 0u) << k;
+                        mask |= (pair_accepted(pi, q3, k2) ? 8u : 0u) << k;
+                    }
+                    for (; k < cnt; ++k)
+                        mask |= (pair_accepted(pi, P.pos[b0 + k], k2) ? 1u : 0u) << k;
+                    while (mask) {
+                        const int j = b0 + __builtin_ctz(mask);
+                        mask &= mask - 1u;
+                        force_pair(pi, ei, P.pos[j], P.ev[j], P.c, k2lo, A);
+                    }
+                }
             }
             force_store(P, i, pi.w, A);
         } else {
@@ -358,7 +427,14 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                         le[hl] = P.ev[b0 + hl];
                     }
                     wave_sync_lds();
-                    for (int k = 0; k < cnt; ++k) force_term(pi, ei, lp[k], le[k], P.c, A);
+                    uint32_t mask = 0;
+                    for (int k = 0; k < cnt; ++k)
+                        mask |= (pair_accepted(pi, lp[k], k2) ? 1u : 0u) << k;
+                    while (mask) {
+                        const int k = __builtin_ctz(mask);
+                        mask &= mask - 1u;
+                        force_pair(pi, ei, lp[k], le[k], P.c, k2lo, A);
+                    }
                     wave_sync_lds();
                 }
             }
@@ -482,7 +558,18 @@ __global__ void k_checksum_final(const double *partial, int nblocks, double *out
     out[threadIdx.x] = t;
 }
 
+// Plain float4 copy: calibration reference for HBM byte counters.
+__global__ __launch_bounds__(kNT) void k_copy(const float4 *src, float4 *dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (size_t)gridDim.x * kNT)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s) {
+    const int blocks = (int)std::min<size_t>((n + kNT - 1) / kNT, 8192);
+    if (blocks > 0) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kNT), 0, s, src, dst, n);
+}
 
 void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s) {
     if (batch)

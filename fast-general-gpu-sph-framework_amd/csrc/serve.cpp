@@ -255,7 +255,10 @@ int gsph_server_run(gsph_server *s, int realtime) {
         // 2. re-compose
         if (changed) {
             old_slots.clear();
-            for (int id : s->active) old_slots.push_back(s->reqs[id].slot);
+            for (int id : s->active) {
+                const int slot = s->reqs[id].slot;
+                old_slots.push_back(slot < 0 ? -1 : engine_scene_begin(s->ctx, slot));
+            }
             int rc = recompose(s, old_slots);
             if (rc) return rc;
             changed = false;

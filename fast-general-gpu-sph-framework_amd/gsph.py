@@ -143,6 +143,7 @@ _SIGS = {
     "gsph_kernel_timing": (_I, [_P, _I]),
     "gsph_kernel_times": (_I, [_P, C.c_char_p, _I, _P, _P, _I, C.POINTER(_I)]),
     "gsph_inject_fault": (_I, [_P, _I]),
+    "gsph_probe_copy": (_I, [C.c_double, _I, C.POINTER(C.c_float)]),
     "gsph_server_create": (_I, [C.POINTER(Params), C.POINTER(Options), C.c_double, _I,
                                 C.POINTER(_P)]),
     "gsph_server_destroy": (_I, [_P]),
@@ -210,6 +211,13 @@ def device_count() -> int:
     n = C.c_int(0)
     _check(load().gsph_device_count(C.byref(n)))
     return n.value
+
+
+def probe_copy(nbytes: float, reps: int = 10) -> float:
+    """Average ms of a plain copy kernel over nbytes (HBM counter calibration)."""
+    ms = C.c_float()
+    _check(load().gsph_probe_copy(float(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def scene_config(k: int):
@@ -343,8 +351,11 @@ class System:
         tasks = np.zeros(max(T + (T & 1), 1), TASK_DTYPE)
         self._c(self._lib.gsph_read_grid(self.h, _ptr(hash_), _ptr(perm), _ptr(begin),
                                          _ptr(count), _ptr(req), _ptr(compact), _ptr(tasks)))
-        return dict(hash=hash_, perm=perm, begin=begin, count=count, req=req, compact=compact,
-                    tasks=tasks[:T + (T & 1)], **info)
+        out = dict(info)
+        out["num_tasks"] = out.pop("tasks")
+        out.update(hash=hash_, perm=perm, begin=begin, count=count, req=req, compact=compact,
+                   tasks=tasks[:T + (T & 1)])
+        return out
 
     def checksum(self, first=None, count=None):
         out = np.zeros(5, np.float64)

@@ -262,6 +262,10 @@ int gsph_get_pair_stats(gsph_ctx *ctx, gsph_pair_stats *out);
 int gsph_kernel_timing(gsph_ctx *ctx, int enable);
 int gsph_kernel_times(gsph_ctx *ctx, char *names, int names_cap, float *ms,
                       int *launches, int cap, int *count);
+/* Measurement helper with no counterpart in the reference: a plain float4
+ * copy kernel over `bytes` bytes, run reps times (average ms returned).  Its
+ * HBM byte counters calibrate the traffic figures of the other kernels. */
+int gsph_probe_copy(double bytes, int reps, float *ms);
 /* Test hook: the next call that launches kernels fails with GSPH_ERR_INJECTED
  * after launching nothing (failure-path tests of the serving loop). */
 int gsph_inject_fault(gsph_ctx *ctx, int after_calls);

@@ -567,7 +567,7 @@ int orc_density_brute_f64(const orc_params *p, int n, const float *pos4,
 // to: the same sums taken over |term|, with the pressure of each particle
 // replaced by its uncancelled part gas*(rho/rho0)^7 (the "- 1" of the state
 // equation cancels, so an FP32 p carries an absolute error relative to that
-// part), plus |surface force|.
+// part), plus the surface force and its per-neighbour contributions.
 int orc_force_f64(const orc_params *p, int n, const float *pos4,
                   const float *ev4, const double *rho, const double *pressure,
                   double *acc, double *acc_scale) {
@@ -582,7 +582,7 @@ int orc_force_f64(const orc_params *p, int n, const float *pos4,
 #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < n; ++i) {
         double P[3] = {0, 0, 0}, V[3] = {0, 0, 0}, G[3] = {0, 0, 0}, L = 0;
-        double Ps = 0, Vs = 0;
+        double Ps = 0, Vs = 0, Gs = 0, Ls = 0;
         const double evi[3] = {ev4[4 * i], ev4[4 * i + 1], ev4[4 * i + 2]};
         for_neighbours(ng, pos4, i, [&](int j, double rx, double ry, double rz,
                                         double r2) {
@@ -602,7 +602,9 @@ int orc_force_f64(const orc_params *p, int n, const float *pos4,
             G[0] += rx * w * t * t;
             G[1] += ry * w * t * t;
             G[2] += rz * w * t * t;
+            Gs += r * w * t * t;
             L += w * t * (r2 - 0.75 * t);
+            Ls += std::fabs(w * t * (r2 - 0.75 * t));
         });
         double F[3];
         for (int a = 0; a < 3; ++a) F[a] = P[a] * d.half_spiky + V[a] * d.visc_visco;
@@ -614,7 +616,13 @@ int orc_force_f64(const orc_params *p, int n, const float *pos4,
             for (int a = 0; a < 3; ++a) S[a] = G[a] * d.surface_coe * L / gn;
         for (int a = 0; a < 3; ++a) acc[3 * i + a] = F[a] * d.mass + S[a];
         if (acc_scale) {
-            const double smag = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+            // surface force G coe L / |G|: its per-neighbour contributions are
+            // coe |L_j| (the Laplacian sum cancels between r < 0.655 h and
+            // r > 0.655 h) and coe |L| |G_j| / |G| (its direction)
+            double smag = std::sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+            if (gn > d.surface_normal)
+                smag += d.surface_coe * (Ls * d.lplc_scale / rho[i] +
+                                         std::fabs(L) * Gs * std::fabs(d.grad_scale) / gn);
             acc_scale[i] = d.mass * (std::fabs(d.half_spiky) * Ps +
                                      d.visc_visco * Vs) +
                            smag;
