@@ -163,7 +163,15 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--legs", default="plain,batched,serving")
+    ap.add_argument("--probe-copy", type=float, default=0.0,
+                    help="only run a plain copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
+    if args.probe_copy > 0:
+        import gsph
+        ms = gsph.probe_copy(args.probe_copy, 5)
+        print(json.dumps({"probe_copy_bytes": args.probe_copy, "avg_ms": ms,
+                          "gbs": 2 * args.probe_copy / (ms * 1e-3) / 1e9}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
               "the path shards by space across GPUs by design, but no multi-GPU "

@@ -295,17 +295,17 @@ __global__ __launch_bounds__(kNT) void k_cells(ArrangeArgs a) {
             const int prev = i == 0 ? -1 : (int)(a.hash[i - 1] >> 6);
             a.cells[k] = cell;
             a.cstart[k] = i;
-            // chunks whose first cell lies in (prev, cell] start at k
-            const int q0 = prev < 0 ? 0 : prev / kFillChunk + 1;
-            const int q1 = cell / kFillChunk;
-            for (int q = q0; q <= q1; ++q) a.chunk_k0[q] = k;
+            // interior chunks whose first cell lies in (prev, cell] start at k;
+            // chunks before the first / after the last occupied cell are
+            // resolved by k_fill itself
+            if (prev >= 0)
+                for (int q = prev / kFillChunk + 1; q <= cell / kFillChunk; ++q) a.chunk_k0[q] = k;
         }
         if (i == a.n - 1) {
             // the last particle closes the list (it sees every boundary before it)
             const int K = (int)(s_base[s][wave] + (uint32_t)__popcll(m & lanemask_lt())) + (b ? 1 : 0);
             a.cstart[K] = a.n;
             a.dc[kCntK] = K;
-            for (int q = cell / kFillChunk + 1; q < a.nchunks; ++q) a.chunk_k0[q] = K;
         }
     }
 }
@@ -336,7 +336,36 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
     const int tid = threadIdx.x;
     const int q = blockIdx.x;
     const int K = a.dc[kCntK];
-    int kcur = a.chunk_k0[q];
+    int kcur;
+    {
+        const int qb = q * kFillChunk;
+        if (K == 0 || qb <= a.cells[0])
+            kcur = 0;                      // at or before the first occupied cell
+        else if (qb > a.cells[K - 1])
+            kcur = K;                      // after the last one
+        else
+            kcur = a.chunk_k0[q];          // written by k_cells
+    }
+    // empty chunk (most of a sparsely filled box): begin is the next
+    // occupied cell's start, count is 0 - no map, no barriers
+    const int chunk_end = min(q * kFillChunk + kFillChunk, a.numc);
+    if (kcur >= K || a.cells[kcur] >= chunk_end) {
+        const int bg = kcur < K ? a.cstart[kcur] : a.n;
+        const int4 b4 = make_int4(bg, bg, bg, bg), z4 = make_int4(0, 0, 0, 0);
+        for (int c0 = q * kFillChunk + 4 * tid; c0 < chunk_end; c0 += 4 * kNT) {
+            if (c0 + 3 < a.numc) {
+                *reinterpret_cast<int4 *>(a.begin + c0) = b4;
+                *reinterpret_cast<int4 *>(a.count + c0) = z4;
+            } else {
+                for (int u = 0; u < 4 && c0 + u < a.numc; ++u) {
+                    a.begin[c0 + u] = bg;
+                    a.count[c0 + u] = 0;
+                }
+            }
+        }
+        if (q == a.nchunks - 1 && tid == 0) a.begin[a.numc] = a.n;
+        return;
+    }
     for (int it = 0; it < kFillChunk / kRound; ++it) {
         const int base = q * kFillChunk + it * kRound;
         if (base >= a.numc) break;

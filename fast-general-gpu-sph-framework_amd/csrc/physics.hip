@@ -139,21 +139,51 @@ __device__ __forceinline__ ItemMap item_map(int items) {
     return m;
 }
 
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+constexpr int kStage = 128;   // candidates staged per wave and round
+constexpr int kNone = 0x7fffffff;
+
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The reference's pair test 1e-12 <= r^2 <= h^2 as one unsigned range test
+// on bit patterns: r^2 is a sum of squares (never negative, never -0), for
+// non-negative floats the bit patterns order like the values, and NaN and
+// +inf fall outside the range.  The host rejects h^2 < 1e-12.
+struct PairRange {
+    uint32_t lo, span;
+};
+
+__device__ __forceinline__ PairRange pair_range(float k2) {
+    return {__float_as_uint(kFloatSmall), __float_as_uint(k2) - __float_as_uint(kFloatSmall)};
+}
+
+__device__ __forceinline__ float dist2(const float4 &a, const float4 &b) {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ bool in_range(float d2, PairRange pr) {
+    return __float_as_uint(d2) - pr.lo <= pr.span;
+}
+
 // ---------------------------------------------------------------------------
 // Density.  rho = m*poly6 * sum (h^2 - r^2)^3 + self_density over
 // 1e-12 <= r^2 <= h^2; w = 1/rho into pos.w, p = ((rho/rho0)^7 - 1)*gas into
 // ev.w (density_kernels.cu:353-358, 536-544).
-__device__ __forceinline__ float density_term(float4 a, float4 b, float h2) {
-    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    const float d2 = dx * dx + dy * dy + dz * dz;
+__device__ __forceinline__ float density_term(const float4 &a, const float4 &b, float h2,
+                                              PairRange pr) {
+    const float d2 = dist2(a, b);
     const float t = h2 - d2;
-    return (d2 >= kFloatSmall && d2 <= h2) ? t * t * t : 0.f;
+    return in_range(d2, pr) ? t * t * t : 0.f;
 }
 
 __device__ __forceinline__ void density_store(const PhysicsArgs &P, int i, float s) {
@@ -165,9 +195,50 @@ __device__ __forceinline__ void density_store(const PhysicsArgs &P, int i, float
         (powf7(rho * c.inv_rest_density) - 1.f) * c.gas_constant;
 }
 
+// The rows of one TRA lane: live flag and sorted range per row.
+template <bool kBatch>
+__device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi, bool active,
+                                         int i, int lo[9], int hi[9]) {
+    const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+    int mx, my, mz;
+    micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
+    RowPlan rp;
+    plan_rows(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, P.row_prune != 0, rp);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        lo[r] = hi[r] = 0;
+        if (active && rp.live[r])
+            row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r], rp.gr[r],
+                      lo[r], hi[r]);
+    }
+}
+
+// Wave-cooperative walk over one row: the lanes' ranges [lo, hi) are covered
+// by rounds of kStage consecutive particles loaded into LDS with coalesced
+// loads (gaps between lanes' ranges are skipped); each lane then reads its
+// own candidates from LDS in increasing index order.  visit(jb, je, base)
+// is called on every lane with [jb, je) = [lo, hi) clipped to
+// [base, base + kStage) (empty for lanes without candidates this round).
+template <class Stage, class Visit>
+__device__ __forceinline__ void staged_row(int lo, int hi, int lane, Stage &&stage,
+                                           Visit &&visit) {
+    int cur = wave_min(lo < hi ? lo : kNone);
+    while (cur != kNone) {
+        stage(cur);
+        wave_sync_lds();
+        const int jb = max(lo, cur), je = min(hi, cur + kStage);
+        visit(jb, je, cur);
+        wave_sync_lds();
+        const int nxt = hi > cur + kStage ? max(lo, cur + kStage) : kNone;
+        cur = wave_min(nxt);
+    }
+    (void)lane;
+}
+
 template <bool kBatch>
 __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     __shared__ float4 s_nb[kNT / 64][64];
+    __shared__ float4 s_stage[kNT / 64][kStage];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
@@ -175,41 +246,44 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const float h2 = P.c.kernel_2;
+    const PairRange pr = pair_range(h2);
+    float4 *stg = s_stage[wave];
     for (int it = m.first; it < m.end; it += m.stride) {
         if (it < tra_tiles) {
+            // all lanes stay in the wave (the row walk is cooperative);
+            // lanes past the TRA range carry empty rows
             const int idx = it * 64 + lane;
-            if (idx >= tra) continue;
-            const int i = P.compact[idx];
-            const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+            const bool active = idx < tra;
+            const int i = P.compact[active ? idx : tra - 1];
             const float4 pi = P.pos[i];
-            int mx, my, mz;
-            micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
-            RowPlan rp;
-            plan_rows(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, P.row_prune != 0, rp);
+            int lo[9], hi[9];
+            tra_rows<kBatch>(P, pi, active, i, lo, hi);
             float s = 0.f;
-#pragma unroll
             for (int r = 0; r < 9; ++r) {
-                if (!rp.live[r]) continue;
-                int lo, hi;
-                row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r],
-                          rp.gr[r], lo, hi);
-                // per-row partial sum (halves the rounding of ~100-term sums);
-                // rejected candidates add exact zeros, so pruning cannot
-                // change the result
+                // per-row partial sum; rejected candidates add exact zeros, so
+                // pruning cannot change the result
                 float rs = 0.f;
-                int j = lo;
-                for (; j + 4 <= hi; j += 4) {
-                    const float4 q0 = P.pos[j], q1 = P.pos[j + 1];
-                    const float4 q2 = P.pos[j + 2], q3 = P.pos[j + 3];
-                    rs += density_term(pi, q0, h2);
-                    rs += density_term(pi, q1, h2);
-                    rs += density_term(pi, q2, h2);
-                    rs += density_term(pi, q3, h2);
-                }
-                for (; j < hi; ++j) rs += density_term(pi, P.pos[j], h2);
+                staged_row(
+                    lo[r], hi[r], lane,
+                    [&](int base) {
+#pragma unroll
+                        for (int k = lane; k < kStage; k += 64)
+                            if (base + k < P.n) stg[k] = P.pos[base + k];
+                    },
+                    [&](int jb, int je, int base) {
+                        const int cnt = je - jb;
+                        if (cnt <= 0) return;
+                        const float4 *q = stg + (jb - base);
+                        int k = 0;
+                        for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) rs += density_term(pi, q[k + u], h2, pr);
+                        }
+                        for (; k < cnt; ++k) rs += density_term(pi, q[k], h2, pr);
+                    });
                 s += rs;
             }
-            density_store(P, i, s);
+            if (active) density_store(P, i, s);
         } else {
             const int task = ((it - tra_tiles) << 1) + (lane >> 5);
             const int hl = lane & 31;
@@ -241,7 +315,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                     const int cnt = min(32, hi - b0);
                     if (hl < cnt) lds[hl] = P.pos[b0 + hl];
                     wave_sync_lds();
-                    for (int k = 0; k < cnt; ++k) rs += density_term(pi, lds[k], h2);
+                    for (int k = 0; k < cnt; ++k) rs += density_term(pi, lds[k], h2, pr);
                     wave_sync_lds();
                 }
                 s += rs;
@@ -257,45 +331,31 @@ struct ForceAcc {
     float px, py, pz, vx, vy, vz, gx, gy, gz, l;
 };
 
-// h^2 - r^2 to ~1 ulp of the result.  Near r = h the plain FP32 difference
-// cancels (its error is ~1 ulp of h^2, i.e. relative h^2/(h^2 - r^2)), which
-// makes the kr^2 and (h^2 - r^2)^2 terms of a pair near the kernel edge
-// inaccurate.  The squares are split exactly (r_x^2 = a + ea by FMA), their
-// sum carried as a double-float (TwoSum), and h^2 = fl(h^2) + k2lo exactly.
-// Coordinates differences are exact (Sterbenz) for neighbours.  FP32 only.
-__device__ __forceinline__ float h2_minus_r2(float rx, float ry, float rz, float k2,
-                                             float k2lo) {
-    const float a = __fmul_rn(rx, rx), b = __fmul_rn(ry, ry), c = __fmul_rn(rz, rz);
-    const float ea = fmaf(rx, rx, -a), eb = fmaf(ry, ry, -b), ec = fmaf(rz, rz, -c);
-    const float s1 = __fadd_rn(a, b);
-    const float z1 = __fsub_rn(s1, a);
-    const float e1 = __fadd_rn(__fsub_rn(a, __fsub_rn(s1, z1)), __fsub_rn(b, z1));
-    const float s = __fadd_rn(s1, c);
-    const float z2 = __fsub_rn(s, s1);
-    const float e2 = __fadd_rn(__fsub_rn(s1, __fsub_rn(s, z2)), __fsub_rn(c, z2));
-    const float lo = __fadd_rn(__fadd_rn(ea, eb), __fadd_rn(ec, __fadd_rn(e1, e2)));
-    return __fadd_rn(__fsub_rn(k2, s), __fsub_rn(k2lo, lo));
+// h^2 - r^2 rounded once.  Near r = h the plain FP32 difference cancels (its
+// error is ~1 ulp of h^2, i.e. relative h^2/(h^2 - r^2)), which makes the
+// kr^2 and (h^2 - r^2)^2 terms of a pair near the kernel edge inaccurate.
+// The coordinate differences of neighbours are exact (Sterbenz), their
+// squares are exact in FP64 and their sum is within 2^-52 r^2; h^2 is exact
+// in FP64 for an FP32 h, so the difference is correct to FP32 rounding.
+__device__ __forceinline__ float h2_minus_r2(float rx, float ry, float rz, double h2d) {
+    const double x = rx, y = ry, z = rz;
+    return (float)(h2d - fma(x, x, fma(y, y, z * z)));
 }
 
-// The reference's acceptance test of a pair: 1e-12 <= r^2 <= h^2 in FP32.
-__device__ __forceinline__ bool pair_accepted(const float4 &pi, const float4 &pj, float k2) {
-    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-    const float d2 = rx * rx + ry * ry + rz * rz;
-    return d2 >= kFloatSmall && d2 <= k2;
-}
 
 // Pressure, viscosity and colour-field terms of one accepted pair.
 __device__ __forceinline__ void force_pair(const float4 &pi, const float4 &ei,
                                            const float4 &pj, const float4 &ej,
-                                           const gsph_constants &c, float k2lo,
+                                           const gsph_constants &c, double h2d,
                                            ForceAcc &A) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float d2 = rx * rx + ry * ry + rz * rz;
     const float inv = __frsqrt_rn(d2);
     const float dist = d2 * inv;
     const float V = pj.w;
-    const float t = h2_minus_r2(rx, ry, rz, c.kernel_2, k2lo);
-    const float kr = t / (c.kernel_h + dist);           // h - r without cancellation
+    const float t = h2_minus_r2(rx, ry, rz, h2d);
+    // h - r without cancellation; the reciprocal is within 1 ulp
+    const float kr = t * __builtin_amdgcn_rcpf(c.kernel_h + dist);
     const float Vkr = V * kr;
     const float pw = Vkr * (ei.w + ej.w) * kr * inv;
     A.px -= rx * pw;
@@ -341,57 +401,66 @@ template <bool kBatch>
 __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     __shared__ float4 s_pos[kNT / 64][64];
     __shared__ float4 s_ev[kNT / 64][64];
+    __shared__ float4 s_stage_p[kNT / 64][kStage];
+    __shared__ float4 s_stage_e[kNT / 64][kStage];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float4 *sp = s_stage_p[wave];
+    float4 *se = s_stage_e[wave];
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
     const int tra_tiles = (tra + 63) >> 6;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
-    const float k2lo = fmaf(P.c.kernel_h, P.c.kernel_h, -P.c.kernel_2);  // h^2 - fl(h^2)
-    const float k2 = P.c.kernel_2;
+    const double h2d = (double)P.c.kernel_h * (double)P.c.kernel_h;   // exact
+    const PairRange pr = pair_range(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         if (it < tra_tiles) {
             const int idx = it * 64 + lane;
-            if (idx >= tra) continue;
-            const int i = P.compact[idx];
-            const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+            const bool active = idx < tra;
+            const int i = P.compact[active ? idx : tra - 1];
             const float4 pi = P.pos[i];
             const float4 ei = P.ev[i];
-            int mx, my, mz;
-            micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
-            RowPlan rp;
-            plan_rows(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, P.row_prune != 0, rp);
+            int lo[9], hi[9];
+            tra_rows<kBatch>(P, pi, active, i, lo, hi);
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
             for (int r = 0; r < 9; ++r) {
-                if (!rp.live[r]) continue;
-                int lo, hi;
-                row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r],
-                          rp.gr[r], lo, hi);
-                // phase 1 tests up to 32 candidates into a mask, phase 2 runs the
-                // pair body over the accepted ones only (in candidate order)
-                for (int b0 = lo; b0 < hi; b0 += 32) {
-                    const int cnt = min(32, hi - b0);
-                    uint32_t mask = 0;
-                    int k = 0;
-                    for (; k + 4 <= cnt; k += 4) {
-                        const float4 q0 = P.pos[b0 + k], q1 = P.pos[b0 + k + 1];
-                        const float4 q2 = P.pos[b0 + k + 2], q3 = P.pos[b0 + k + 3];
-                        mask |= (pair_accepted(pi, q0, k2) ? 1u : 0u) << k;
-                        mask |= (pair_accepted(pi, q1, k2) ? 2u : 0u) << k;
-                        mask |= (pair_accepted(pi, q2, k2) ? 4u : 0u) << k;
-                        mask |= (pair_accepted(pi, q3, k2) ? 8u : 0u) << k;
-                    }
-                    for (; k < cnt; ++k)
-                        mask |= (pair_accepted(pi, P.pos[b0 + k], k2) ? 1u : 0u) << k;
-                    while (mask) {
-                        const int j = b0 + __builtin_ctz(mask);
-                        mask &= mask - 1u;
-                        force_pair(pi, ei, P.pos[j], P.ev[j], P.c, k2lo, A);
-                    }
-                }
+                staged_row(
+                    lo[r], hi[r], lane,
+                    [&](int base) {
+#pragma unroll
+                        for (int k = lane; k < kStage; k += 64)
+                            if (base + k < P.n) {
+                                sp[k] = P.pos[base + k];
+                                se[k] = P.ev[base + k];
+                            }
+                    },
+                    [&](int jb, int je, int base) {
+                        // phase 1 tests up to 32 candidates into a mask, phase 2
+                        // runs the pair body over the accepted ones (in order)
+                        for (int b0 = jb; b0 < je; b0 += 32) {
+                            const int cnt = min(32, je - b0);
+                            const float4 *q = sp + (b0 - base);
+                            const float4 *qe = se + (b0 - base);
+                            // bits enter at the bottom (reversed order, undone below)
+                            uint32_t mask = 0;
+                            int k = 0;
+                            for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+                                for (int u = 0; u < 4; ++u)
+                                    mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
+                            }
+                            for (; k < cnt; ++k)
+                                mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
+                            mask = __builtin_bitreverse32(mask) >> (32 - cnt);
+                            while (mask) {
+                                const int kk = __builtin_ctz(mask);
+                                mask &= mask - 1u;
+                                force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
+                            }
+                        }
+                    });
             }
-            force_store(P, i, pi.w, A);
+            if (active) force_store(P, i, pi.w, A);
         } else {
             const int task = ((it - tra_tiles) << 1) + (lane >> 5);
             const int hl = lane & 31;
@@ -429,11 +498,11 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                     wave_sync_lds();
                     uint32_t mask = 0;
                     for (int k = 0; k < cnt; ++k)
-                        mask |= (pair_accepted(pi, lp[k], k2) ? 1u : 0u) << k;
+                        mask |= (in_range(dist2(pi, lp[k]), pr) ? 1u : 0u) << k;
                     while (mask) {
                         const int k = __builtin_ctz(mask);
                         mask &= mask - 1u;
-                        force_pair(pi, ei, lp[k], le[k], P.c, k2lo, A);
+                        force_pair(pi, ei, lp[k], le[k], P.c, h2d, A);
                     }
                     wave_sync_lds();
                 }

@@ -224,8 +224,9 @@ void gsph_options_default(gsph_options *out) {
 
 int gsph_derive_constants(const gsph_params *p, gsph_constants *c) {
     if (!p || !c) return gsph::fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
-    if (!(p->kernel_h > 0.f))
-        return gsph::fail(nullptr, GSPH_ERR_INVALID_ARG, "kernel_h must be > 0");
+    // h^2 >= 1e-12: below it the reference's pair test accepts nothing
+    if (!(p->kernel_h >= 1e-6f) || !(p->kernel_h * p->kernel_h >= 1e-12f))
+        return gsph::fail(nullptr, GSPH_ERR_INVALID_ARG, "kernel_h must be >= 1e-6");
     std::memset(c, 0, sizeof(*c));
     const float h = p->kernel_h;
     c->kernel_h = h;

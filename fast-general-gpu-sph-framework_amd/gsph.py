@@ -14,7 +14,7 @@ from pathlib import Path
 import numpy as np
 
 PKG_DIR = Path(__file__).resolve().parent
-LIB_PATH = PKG_DIR / "_build" / "libgsph.so"
+LIB_PATH = Path(os.environ.get("GSPH_LIBRARY", PKG_DIR / "_build" / "libgsph.so"))
 
 ERRORS = {
     -1: "GSPH_ERR_INVALID_ARG",

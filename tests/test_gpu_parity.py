@@ -126,9 +126,20 @@ def test_one_step_physics(gsph, po, name):
     ref = po.fp64_step(op, pos, np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32))
     errs = step_errors(pos1, vel1, ev1, acc1, ref, op)
     print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    if max(errs.values()) > REL_TOL:
+        _dump(f"step_{name}", pos=pos, pos1=pos1, vel1=vel1, ev1=ev1, acc1=acc1)
     for k, v in errs.items():
         assert v <= REL_TOL, f"{k} error {v:.3e} > {REL_TOL}"
     s.close()
+
+
+def _dump(tag, **arrays):
+    """Keep the GPU output of a failing numerical case for offline analysis."""
+    import os
+    from pathlib import Path
+    out = Path(os.environ.get("GRAFT_REPO_ROOT", ".")) / "gpurun_out"
+    if out.is_dir():
+        np.savez_compressed(out / f"{tag}.npz", **arrays)
 
 
 def test_step_from_moving_state(gsph, po):
