@@ -63,6 +63,7 @@ struct gsph_ctx {
     int *begin = nullptr, *count = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
     uint2 *xcum = nullptr;
     int *compact = nullptr;
+    uint32_t *nbmask = nullptr;   // [10][cap_n] accepted-pair masks
     gsph_task *tasks = nullptr;
     int *dc = nullptr;
     SceneGeom *scenes_d = nullptr;
@@ -71,6 +72,7 @@ struct gsph_ctx {
     // state tracking
     int cur = 0;              // buffer holding the current state (0 = A, 1 = B)
     bool grid_valid = false;  // arrange outputs match the state in B
+    bool masks_valid = false; // nbmask holds the pair masks of this arrangement
     // graph replay of one full step
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -250,6 +252,9 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.dc = ctx->dc;
     p.grid_blocks = ctx->grid_blocks;
     p.row_prune = ctx->opt.row_prune;
+    p.nbmask = ctx->nbmask;
+    p.mask_stride = ctx->cap_n;
+    p.use_masks = 0;
     return p;
 }
 
@@ -280,19 +285,24 @@ int do_arrange(gsph_ctx *ctx) {
     if (rc) return rc;
     ctx->cur = 1;
     ctx->grid_valid = true;
+    ctx->masks_valid = false;
     return GSPH_OK;
 }
 
 int do_density(gsph_ctx *ctx) {
     if (ctx->n == 0) return GSPH_OK;
-    const PhysicsArgs p = physics_args(ctx);
+    PhysicsArgs p = physics_args(ctx);
+    p.use_masks = ctx->opt.pair_masks != 0;
     { Timed t(ctx, "density"); launch_density(p, ctx->batch, ctx->stream); }
-    return check_launch(ctx, "density");
+    const int rc = check_launch(ctx, "density");
+    ctx->masks_valid = !rc && p.use_masks;
+    return rc;
 }
 
 int do_force(gsph_ctx *ctx) {
     if (ctx->n == 0) return GSPH_OK;
-    const PhysicsArgs p = physics_args(ctx);
+    PhysicsArgs p = physics_args(ctx);
+    p.use_masks = ctx->masks_valid && ctx->opt.pair_masks != 0;
     { Timed t(ctx, "force"); launch_force(p, ctx->batch, ctx->stream); }
     return check_launch(ctx, "force");
 }
@@ -306,6 +316,7 @@ int do_integrate(gsph_ctx *ctx) {
     }
     ctx->cur = 0;
     ctx->grid_valid = false;
+    ctx->masks_valid = false;
     return GSPH_OK;
 }
 
@@ -386,6 +397,7 @@ int configure_layout(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n) {
         ctx->graph = nullptr;
     }
     ctx->grid_valid = false;
+    ctx->masks_valid = false;
     return GSPH_OK;
 }
 
@@ -428,6 +440,7 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->sms_base, C);
     ALLOC(ctx->xcum, C);
     ALLOC(ctx->compact, N);
+    ALLOC(ctx->nbmask, 10 * N);
     ALLOC(ctx->tasks, task_cap);
     ALLOC(ctx->dc, kCntSlots);
     ALLOC(ctx->scenes_d, ctx->cap_s);
@@ -566,8 +579,9 @@ namespace gsph {
 
 size_t engine_bytes(long long n, long long cells) {
     // per particle: 2 x state (44 B), acc, 2 x key/value, cell list, compact,
-    // task slot; per cell: begin, count, req, TRA/SMS bases, x-layer starts
-    return (size_t)(n * 196 + cells * 28 + (1 << 20));
+    // task slot, pair masks; per cell: begin, count, req, TRA/SMS bases,
+    // x-layer starts
+    return (size_t)(n * 236 + cells * 28 + (1 << 20));
 }
 
 int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,
@@ -786,6 +800,7 @@ int gsph_set_state(gsph_ctx *ctx, const float *pos4, const float *vel3, const fl
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->cur = 0;
     ctx->grid_valid = false;
+    ctx->masks_valid = false;
     return GSPH_OK;
 }
 
@@ -857,6 +872,7 @@ int gsph_step(gsph_ctx *ctx, int nsteps, gsph_stage_ms *per_step_ms) {
             HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
             ctx->cur = 0;
             ctx->grid_valid = false;
+            ctx->masks_valid = false;
         } else {
             rc = do_step(ctx);
             if (rc) return rc;

@@ -87,6 +87,11 @@ struct PhysicsArgs {
     const int *dc;
     int grid_blocks;       // persistent grid of density / force
     int row_prune;
+    // accepted-pair masks of TRA particles: 10 words per entry of compact,
+    // word-major with stride mask_stride; written by density, read by force
+    uint32_t *nbmask;
+    int mask_stride;
+    int use_masks;
 };
 
 // arrange.hip

@@ -145,6 +145,12 @@ __device__ __forceinline__ int wave_min(int v) {
     return v;
 }
 
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 constexpr int kStage = 128;   // candidates staged per wave and round
 constexpr int kNone = 0x7fffffff;
 
@@ -166,9 +172,21 @@ __device__ __forceinline__ PairRange pair_range(float k2) {
     return {__float_as_uint(kFloatSmall), __float_as_uint(k2) - __float_as_uint(kFloatSmall)};
 }
 
+// r^2 with explicit FMAs, so that every kernel evaluates the pair test with
+// the same roundings (the force kernel consumes the density kernel's test).
 __device__ __forceinline__ float dist2(const float4 &a, const float4 &b) {
     const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    return dx * dx + dy * dy + dz * dz;
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// Accepted-pair masks of TRA particles: word r < 9 of entry idx holds bit k
+// for candidate lo_r + k of row r (rows of at most 32 candidates); word 9 is
+// nonzero when some row is longer, and force then tests that particle itself.
+constexpr int kMaskRowMax = 32;
+
+__device__ __forceinline__ uint32_t finish_mask(uint32_t shifted, int cnt) {
+    // bits entered at the bottom in candidate order; reverse into bit k
+    return cnt > 0 ? __builtin_bitreverse32(shifted) >> (32 - cnt) : 0u;
 }
 
 __device__ __forceinline__ bool in_range(float d2, PairRange pr) {
@@ -184,6 +202,16 @@ __device__ __forceinline__ float density_term(const float4 &a, const float4 &b, 
     const float d2 = dist2(a, b);
     const float t = h2 - d2;
     return in_range(d2, pr) ? t * t * t : 0.f;
+}
+
+// density_term plus the acceptance bit shifted into mk
+__device__ __forceinline__ void density_pair(const float4 &a, const float4 &b, float h2,
+                                             PairRange pr, float &rs, uint32_t &mk) {
+    const float d2 = dist2(a, b);
+    const float t = h2 - d2;
+    const bool ok = in_range(d2, pr);
+    rs += ok ? t * t * t : 0.f;
+    mk += mk + (ok ? 1u : 0u);
 }
 
 __device__ __forceinline__ void density_store(const PhysicsArgs &P, int i, float s) {
@@ -259,10 +287,12 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             int lo[9], hi[9];
             tra_rows<kBatch>(P, pi, active, i, lo, hi);
             float s = 0.f;
+            bool long_row = false;
             for (int r = 0; r < 9; ++r) {
                 // per-row partial sum; rejected candidates add exact zeros, so
                 // pruning cannot change the result
                 float rs = 0.f;
+                uint32_t mk = 0;
                 staged_row(
                     lo[r], hi[r], lane,
                     [&](int base) {
@@ -277,13 +307,23 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                         int k = 0;
                         for (; k + 4 <= cnt; k += 4) {
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) rs += density_term(pi, q[k + u], h2, pr);
+                            for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
                         }
-                        for (; k < cnt; ++k) rs += density_term(pi, q[k], h2, pr);
+                        for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
                     });
                 s += rs;
+                if (P.use_masks) {
+                    const int cnt = hi[r] - lo[r];
+                    long_row |= cnt > kMaskRowMax;
+                    if (active)
+                        P.nbmask[(size_t)r * P.mask_stride + idx] =
+                            cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
+                }
             }
-            if (active) density_store(P, i, s);
+            if (active) {
+                density_store(P, i, s);
+                if (P.use_masks) P.nbmask[(size_t)9 * P.mask_stride + idx] = long_row ? 1u : 0u;
+            }
         } else {
             const int task = ((it - tra_tiles) << 1) + (lane >> 5);
             const int hl = lane & 31;
@@ -399,13 +439,11 @@ __device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, float w
 
 template <bool kBatch>
 __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
-    __shared__ float4 s_pos[kNT / 64][64];
-    __shared__ float4 s_ev[kNT / 64][64];
-    __shared__ float4 s_stage_p[kNT / 64][kStage];
-    __shared__ float4 s_stage_e[kNT / 64][kStage];
+    // one LDS slice per wave: staged rows (pos, ev) or SMS batches
+    __shared__ float4 s_wave[kNT / 64][2 * kStage];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    float4 *sp = s_stage_p[wave];
-    float4 *se = s_stage_e[wave];
+    float4 *sp = s_wave[wave];
+    float4 *se = sp + kStage;
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
     const int tra_tiles = (tra + 63) >> 6;
@@ -414,6 +452,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     const double h2d = (double)P.c.kernel_h * (double)P.c.kernel_h;   // exact
     const PairRange pr = pair_range(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
+        wave_sync_lds();   // the previous item's LDS reads precede this item's writes
         if (it < tra_tiles) {
             const int idx = it * 64 + lane;
             const bool active = idx < tra;
@@ -423,7 +462,19 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             int lo[9], hi[9];
             tra_rows<kBatch>(P, pi, active, i, lo, hi);
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            // lanes whose density pass recorded masks walk their accepted
+            // pairs in one flat loop (rows in order, candidates in order);
+            // the others test their rows here, in the same order
+            // each row is staged through LDS; lanes whose density pass
+            // recorded the row's accepted-pair mask run the pair body over
+            // it, the others (masks off, or a row longer than one mask word)
+            // test the staged candidates first.  Same pairs, same order.
+            const bool self_test =
+                !P.use_masks || (active && P.nbmask[(size_t)9 * P.mask_stride + idx] != 0);
             for (int r = 0; r < 9; ++r) {
+                const uint32_t mr = (active && !self_test)
+                                        ? P.nbmask[(size_t)r * P.mask_stride + idx] : 0u;
+                const int lor = lo[r];
                 staged_row(
                     lo[r], hi[r], lane,
                     [&](int base) {
@@ -435,25 +486,39 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                             }
                     },
                     [&](int jb, int je, int base) {
+                        const float4 *q = sp + (jb - base);
+                        const float4 *qe = se + (jb - base);
+                        if (!self_test) {
+                            const int cnt = je - jb;   // <= 32, and jb - lo < 32
+                            uint32_t mm = 0;
+                            if (cnt > 0) {
+                                mm = mr >> (jb - lor);
+                                if (cnt < 32) mm &= (1u << cnt) - 1u;
+                            }
+                            while (mm) {
+                                const int kk = __builtin_ctz(mm);
+                                mm &= mm - 1u;
+                                force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
+                            }
+                            return;
+                        }
                         // phase 1 tests up to 32 candidates into a mask, phase 2
                         // runs the pair body over the accepted ones (in order)
-                        for (int b0 = jb; b0 < je; b0 += 32) {
-                            const int cnt = min(32, je - b0);
-                            const float4 *q = sp + (b0 - base);
-                            const float4 *qe = se + (b0 - base);
+                        for (int b0 = 0; b0 < je - jb; b0 += 32) {
+                            const int cnt = min(32, je - jb - b0);
                             // bits enter at the bottom (reversed order, undone below)
                             uint32_t mask = 0;
                             int k = 0;
                             for (; k + 4 <= cnt; k += 4) {
 #pragma unroll
                                 for (int u = 0; u < 4; ++u)
-                                    mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
+                                    mask += mask + (in_range(dist2(pi, q[b0 + k + u]), pr) ? 1u : 0u);
                             }
                             for (; k < cnt; ++k)
-                                mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
+                                mask += mask + (in_range(dist2(pi, q[b0 + k]), pr) ? 1u : 0u);
                             mask = __builtin_bitreverse32(mask) >> (32 - cnt);
                             while (mask) {
-                                const int kk = __builtin_ctz(mask);
+                                const int kk = b0 + __builtin_ctz(mask);
                                 mask &= mask - 1u;
                                 force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
                             }
@@ -481,8 +546,8 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             const int gl = 4 * cx - 4 + (has_task ? tk.xxi : 0);
             const int gr = 4 * cx + 4 + (has_task ? tk.xxx : 0);
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            float4 *lp = &s_pos[wave][lane & 32];
-            float4 *le = &s_ev[wave][lane & 32];
+            float4 *lp = s_wave[wave] + (lane & 32);
+            float4 *le = s_wave[wave] + 64 + (lane & 32);
             for (int r = 0; r < 9; ++r) {
                 const int ny = cy + r % 3 - 1, nz = cz + r / 3 - 1;
                 int lo = 0, hi = 0;

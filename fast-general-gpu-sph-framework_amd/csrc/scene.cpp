@@ -218,7 +218,7 @@ void gsph_options_default(gsph_options *out) {
     out->sms_min_self = 15;
     out->sms_min_total = 96;
     out->blocks_per_cu = 8;
-    out->sms_batch = 32;
+    out->pair_masks = 1;
     out->row_prune = 1;
 }
 

@@ -72,7 +72,7 @@ class Constants(C.Structure):
 class Options(C.Structure):
     _fields_ = [
         ("split_mode", C.c_int), ("sms_min_self", C.c_int), ("sms_min_total", C.c_int),
-        ("blocks_per_cu", C.c_int), ("sms_batch", C.c_int), ("row_prune", C.c_int),
+        ("blocks_per_cu", C.c_int), ("pair_masks", C.c_int), ("row_prune", C.c_int),
     ]
 
 

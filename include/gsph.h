@@ -122,7 +122,7 @@ typedef struct gsph_options {
     int sms_min_self;     /* tuned: cell is TRA when n <  sms_min_self ... */
     int sms_min_total;    /* ... and the 7-cell total < sms_min_total */
     int blocks_per_cu;    /* persistent grid size of density/force, per CU */
-    int sms_batch;        /* LDS staging depth of the SMS path: 32 or 64 */
+    int pair_masks;       /* 1: density records accepted pairs for force */
     int row_prune;        /* 1: skip rows/layers farther than h (exact superset) */
 } gsph_options;
 

@@ -194,6 +194,26 @@ def test_determinism_and_prune_equivalence(gsph, po):
         assert np.array_equal(out[0][k], out[2][k])     # pruned rows change nothing
 
 
+ALL_TRA = {"split_mode": 1, "sms_min_self": 1 << 20, "sms_min_total": 1 << 20}
+
+
+@pytest.mark.parametrize("name,opts", [("mixed", {}), ("dense", ALL_TRA)])
+def test_pair_masks_change_nothing(gsph, po, name, opts):
+    """Force walking density's accepted-pair masks gives the same bits as force
+    testing every candidate itself (dense all-TRA rows exceed one mask word,
+    so those particles take the self-testing path inside the same waves)."""
+    op, pos = scene(po, name)
+    banner(f"pair masks {name} {opts}")
+    out = []
+    for masks in (1, 0):
+        s = make_system(gsph, op, pos, pair_masks=masks, **opts)
+        s.step(3)
+        out.append(s.read_state())
+        s.close()
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k])
+
+
 def test_graph_replay_matches_eager(gsph, po):
     op, pos = dense_scene(po)
     banner("graph replay dense")
