@@ -19,6 +19,17 @@ sys.path.insert(0, str(ROOT / "fast-general-gpu-sph-framework_amd"))
 
 METRIC = "particle-steps/s at 3.94M; batched scenes/s + p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 summaries of the plain workload (written with profsum.py)
+TRAFFIC_PROFILE = ROOT / "profiles" / "hbm_traffic_plain_3.94M.md"
+VALU_PROFILE = ROOT / "profiles" / "valu_plain_3.94M.md"
+
+
+def profile_block(path):
+    """The JSON block of a committed rocprof summary, or {} if absent."""
+    if not path.exists():
+        return {}
+    import profsum
+    return profsum.read_json_block(path)
 
 
 def algorithmic_bytes(n, ncell):
@@ -160,7 +171,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--legs", default="plain,batched,serving")
     ap.add_argument("--probe-copy", type=float, default=0.0,
@@ -223,6 +234,11 @@ def main():
                  for k in ("arrange", "density", "force", "integrate")}
     step_gbs = alg["step"] / (ms_per_step * 1e-3) / 1e9
 
+    kname = {"density": "k_density<false>", "force": "k_force<false>",
+             "integrate": "k_integrate<false>"}
+    traffic = profile_block(TRAFFIC_PROFILE).get("traffic", {})
+    valu = profile_block(VALU_PROFILE).get("counters", {})
+    dom_traffic = traffic.get(kname[dom], {}).get("bytes")
     out = {
         "metric": METRIC,
         "value": value,
@@ -241,7 +257,10 @@ def main():
                    "particles": n, "cells": ncell, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": alg[dom], "avg_ms": dom_ms},
+                     "traffic": dom_traffic, "algorithmic_bytes": alg[dom], "avg_ms": dom_ms,
+                     "traffic_source": TRAFFIC_PROFILE.name if dom_traffic else None,
+                     "valu_busy_pct": valu.get(kname[dom], {}).get("valu_busy_pct"),
+                     "valu_util_pct": valu.get(kname[dom], {}).get("valu_util_pct")},
         "step_roofline": {"algorithmic_bytes": alg["step"], "achieved_gbs": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "stages_ms": stage_avg,
