@@ -62,6 +62,10 @@ def cpu_baseline(params, pos, steps, threads):
     }
 
 
+def opt_dict(o):
+    return {k: getattr(o, k) for k, _ in o._fields_}
+
+
 def batch_scenes(gsph, count=256, seed=20261206):
     """Workload 3: `count` dam-break blocks, N log-uniform in [4096, 262144],
     aspect ratios y/x and z/x uniform in [0.5, 2], spacing 0.01, corner 0.06,
@@ -89,6 +93,7 @@ def batched_leg(gsph, steps, warmup, count=256, loop_steps=5):
     counts = [len(b) for b in blocks]
     n = int(sum(counts))
     b = gsph.System(params, np.concatenate(blocks), batch_worlds=worlds, batch_counts=counts)
+    tuned, _ = b.autotune(trial_steps=2)
     b.enable_graph(True)
     b.step(warmup)
     b.sync()
@@ -129,6 +134,7 @@ def batched_leg(gsph, steps, warmup, count=256, loop_steps=5):
         "plain_loop_particle_steps_per_s": loop_ps,
         "plain_loop_scenes_per_s": count * loop_steps / loop_wall,
         "speedup_vs_plain_loop": batched_ps / loop_ps,
+        "autotuned_options": opt_dict(tuned),
     }
 
 
@@ -198,6 +204,9 @@ def main():
     ncell = c.grid_size[0] * c.grid_size[1] * c.grid_size[2]
     print(f"[bench] workload {args.config}: n={n} cells={ncell}", file=sys.stderr, flush=True)
     s = gsph.System(params, pos)
+    tuned, tuned_ms = s.autotune(trial_steps=3)   # launch shape only: same bits
+    print(f"[bench] autotune: {opt_dict(tuned)} ({tuned_ms:.3f} ms/step)", file=sys.stderr,
+          flush=True)
     s.step(args.warmup)
     s.sync()
     print("[bench] warmup done", file=sys.stderr, flush=True)
@@ -267,6 +276,7 @@ def main():
         "stages_gbs": stage_gbs,
         "kernels_ms": {k: v["avg_ms"] for k, v in kern.items()},
         "split": {"tra": info["tra"], "sms_tasks": info["tasks"]},
+        "autotuned_options": opt_dict(tuned),
         "checksum": [float(x) for x in cs],
     }
     s.close()

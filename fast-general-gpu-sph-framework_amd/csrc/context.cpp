@@ -49,6 +49,7 @@ struct gsph_ctx {
     int sort_tiles = 0, cell_tiles = 0, nchunks = 0, split_tiles = 0, task_cap = 0;
     int cap_n = 0, cap_c = 0, cap_s = 0;  // allocated capacity
     int grid_blocks = 0;
+    int cus = 256;            // compute units of the device
     int device = 0;
     hipStream_t stream = nullptr;
     // device buffers
@@ -255,6 +256,7 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.nbmask = ctx->nbmask;
     p.mask_stride = ctx->cap_n;
     p.use_masks = 0;
+    p.stage_depth = ctx->opt.stage_depth;
     return p;
 }
 
@@ -454,6 +456,26 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
 }
 
 // An empty context: constants, stream, events; no buffers yet.
+// Persistent density/force grid from blocks_per_cu: a multiple of 8 blocks
+// so that every XCD gets the same share.
+void set_grid(gsph_ctx *ctx) {
+    const int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 8;
+    ctx->grid_blocks = std::max(8, (ctx->cus * bpc + 7) / 8 * 8);
+}
+
+// Validates o; 0 selects the default of blocks_per_cu and stage_depth.
+int check_options(gsph_ctx *ctx, gsph_options &o) {
+    if (o.blocks_per_cu == 0) o.blocks_per_cu = 8;
+    if (o.stage_depth == 0) o.stage_depth = 128;
+    if (o.stage_depth != 64 && o.stage_depth != 128 && o.stage_depth != 256)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "stage_depth must be 64, 128 or 256");
+    if (o.blocks_per_cu < 1 || o.blocks_per_cu > 32)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "blocks_per_cu must be in [1, 32]");
+    if (o.split_mode != 0 && o.split_mode != 1)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "split_mode must be 0 or 1");
+    return GSPH_OK;
+}
+
 int new_context(const gsph_params *p, const gsph_options *opt, bool batch, gsph_ctx **out) {
     if (!p || !out) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
     int ndev = 0;
@@ -465,6 +487,10 @@ int new_context(const gsph_params *p, const gsph_options *opt, bool batch, gsph_
         ctx->opt = *opt;
     else
         gsph_options_default(&ctx->opt);
+    if (check_options(nullptr, ctx->opt)) {
+        delete ctx;
+        return GSPH_ERR_INVALID_ARG;
+    }
     int rc = gsph_derive_constants(p, &ctx->c);
     if (rc) {
         delete ctx;
@@ -473,10 +499,9 @@ int new_context(const gsph_params *p, const gsph_options *opt, bool batch, gsph_
     ctx->batch = batch;
     hipDeviceProp_t prop;
     (void)hipGetDevice(&ctx->device);
-    int cus = 256;
-    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;
-    const int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 8;
-    ctx->grid_blocks = std::max(8, (cus * bpc + 7) / 8 * 8);
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess)
+        ctx->cus = prop.multiProcessorCount;
+    set_grid(ctx);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return fail(nullptr, GSPH_ERR_HIP, "hipStreamCreate failed");
@@ -878,6 +903,145 @@ int gsph_step(gsph_ctx *ctx, int nsteps, gsph_stage_ms *per_step_ms) {
             if (rc) return rc;
         }
     }
+    return GSPH_OK;
+}
+
+namespace {
+
+// Times `steps` steps of the context's scene from the saved state under
+// options o (one untimed step first); restores the saved state afterwards.
+int time_trial(gsph_ctx *ctx, const gsph_options &o, const float4 *pos0, const f3 *vel0,
+               const float4 *ev0, int steps, float *ms) {
+    ctx->opt = o;
+    set_grid(ctx);
+    const size_t n = (size_t)ctx->n;
+    hipStream_t s = ctx->stream;
+    auto restore = [&]() -> int {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->pos[0], pos0, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->vel[0], vel0, n * sizeof(f3), hipMemcpyDeviceToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->ev[0], ev0, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+        ctx->cur = 0;
+        ctx->grid_valid = false;
+        ctx->masks_valid = false;
+        return GSPH_OK;
+    };
+    int rc = restore();
+    if (!rc) rc = do_step(ctx);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(ctx, GSPH_ERR_HIP, "hipEventCreate failed");
+    if (!rc) {
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < steps && !rc; ++k) rc = do_step(ctx);
+        (void)hipEventRecord(e1, s);
+        if (!rc && hipEventSynchronize(e1) != hipSuccess)
+            rc = fail(ctx, GSPH_ERR_HIP, "trial synchronisation failed");
+        if (!rc) (void)hipEventElapsedTime(ms, e0, e1);
+        *ms /= (float)std::max(steps, 1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (!rc) rc = collect_errors(ctx);
+    const int rc2 = restore();
+    return rc ? rc : rc2;
+}
+
+}  // namespace
+
+int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *chosen,
+                  float *best_ms) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    if (ctx->cur != 0) return fail(ctx, GSPH_ERR_INVALID_ARG, "autotune expects the integrated state");
+    const gsph_options start = ctx->opt;
+    if (ctx->n == 0) {
+        if (chosen) *chosen = start;
+        if (best_ms) *best_ms = 0.f;
+        return GSPH_OK;
+    }
+    const int steps = std::min(std::max(trial_steps, 1), 100);
+    const size_t n = (size_t)ctx->n;
+    float4 *pos0 = nullptr, *ev0 = nullptr;
+    f3 *vel0 = nullptr;
+    HIP_TRY(ctx, hipMalloc(&pos0, n * sizeof(float4)));
+    if (hipMalloc(&vel0, n * sizeof(f3)) != hipSuccess || hipMalloc(&ev0, n * sizeof(float4)) != hipSuccess) {
+        (void)hipFree(pos0);
+        if (vel0) (void)hipFree(vel0);
+        return fail(ctx, GSPH_ERR_HIP, "hipMalloc of the autotune snapshot failed");
+    }
+    hipStream_t s = ctx->stream;
+    int rc = GSPH_OK;
+    if (hipMemcpyAsync(pos0, ctx->pos[0], n * sizeof(float4), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(vel0, ctx->vel[0], n * sizeof(f3), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ev0, ctx->ev[0], n * sizeof(float4), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = fail(ctx, GSPH_ERR_HIP, "autotune snapshot copy failed");
+    // coordinate search: launch shape (grid x staging depth x pair masks)
+    // under the current split, then the split rule under the best shape
+    gsph_options best = start;
+    float best_t = 1e30f;
+    auto trial = [&](const gsph_options &o) {
+        if (rc) return;
+        float t = 0.f;
+        rc = time_trial(ctx, o, pos0, vel0, ev0, steps, &t);
+        if (!rc && t < best_t) {
+            best_t = t;
+            best = o;
+        }
+    };
+    for (int bpc : {4, 8, 16})
+        for (int depth : {64, 128, 256})
+            for (int masks : {1, 0}) {
+                gsph_options o = start;
+                o.blocks_per_cu = bpc;
+                o.stage_depth = depth;
+                o.pair_masks = masks;
+                trial(o);
+            }
+    if (tune_split) {
+        const gsph_options shape = best;
+        gsph_options o = shape;
+        o.split_mode = 0;
+        trial(o);
+        const int rules[][2] = {{15, 96}, {24, 128}, {32, 192}, {48, 256}, {64, 384}};
+        for (const auto &r : rules) {
+            o = shape;
+            o.split_mode = 1;
+            o.sms_min_self = r[0];
+            o.sms_min_total = r[1];
+            trial(o);
+        }
+    }
+    (void)hipFree(pos0);
+    (void)hipFree(vel0);
+    (void)hipFree(ev0);
+    ctx->opt = rc ? start : best;
+    set_grid(ctx);
+    if (ctx->graph_exec) {   // captured with the old launch shape
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
+    if (rc) return rc;
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+    return GSPH_OK;
+}
+
+int gsph_set_options(gsph_ctx *ctx, const gsph_options *opt) {
+    if (!ctx || !opt) return fail(ctx, GSPH_ERR_INVALID_ARG, "null argument");
+    gsph_options o = *opt;
+    int rc = check_options(ctx, o);
+    if (rc) return rc;
+    ctx->opt = o;
+    set_grid(ctx);
+    if (ctx->graph_exec) {
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
+    return GSPH_OK;
+}
+
+int gsph_get_options(gsph_ctx *ctx, gsph_options *out) {
+    if (!ctx || !out) return fail(ctx, GSPH_ERR_INVALID_ARG, "null argument");
+    *out = ctx->opt;
     return GSPH_OK;
 }
 

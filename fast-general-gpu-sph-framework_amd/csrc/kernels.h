@@ -92,6 +92,7 @@ struct PhysicsArgs {
     uint32_t *nbmask;
     int mask_stride;
     int use_masks;
+    int stage_depth;       // LDS staging depth of TRA rows: 64, 128 or 256
 };
 
 // arrange.hip

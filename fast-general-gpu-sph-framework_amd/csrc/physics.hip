@@ -139,19 +139,21 @@ __device__ __forceinline__ ItemMap item_map(int items) {
     return m;
 }
 
+// Wave-wide minimum with DPP lane moves (quad permutes, row mirrors, row
+// broadcasts 15/31), read back from lane 63: no LDS round trips.
 __device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
+    constexpr int kMax = 0x7fffffff;
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0xb1, 0xf, 0xf, false));  // quad [1,0,3,2]
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x4e, 0xf, 0xf, false));  // quad [2,3,0,1]
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x141, 0xf, 0xf, false)); // row half mirror
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x140, 0xf, 0xf, false)); // row mirror
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x142, 0xa, 0xf, false)); // row bcast 15
+    v = min(v, __builtin_amdgcn_update_dpp(kMax, v, 0x143, 0xc, 0xf, false)); // row bcast 31
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-constexpr int kStage = 128;   // candidates staged per wave and round
+// candidates staged per wave and round: a launch option (64, 128 or 256)
+constexpr int kStageDefault = 128;
 constexpr int kNone = 0x7fffffff;
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -247,7 +249,7 @@ __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi,
 // own candidates from LDS in increasing index order.  visit(jb, je, base)
 // is called on every lane with [jb, je) = [lo, hi) clipped to
 // [base, base + kStage) (empty for lanes without candidates this round).
-template <class Stage, class Visit>
+template <int kStage, class Stage, class Visit>
 __device__ __forceinline__ void staged_row(int lo, int hi, int lane, Stage &&stage,
                                            Visit &&visit) {
     int cur = wave_min(lo < hi ? lo : kNone);
@@ -263,7 +265,7 @@ __device__ __forceinline__ void staged_row(int lo, int hi, int lane, Stage &&sta
     (void)lane;
 }
 
-template <bool kBatch>
+template <bool kBatch, int kStage>
 __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     __shared__ float4 s_nb[kNT / 64][64];
     __shared__ float4 s_stage[kNT / 64][kStage];
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 // pruning cannot change the result
                 float rs = 0.f;
                 uint32_t mk = 0;
-                staged_row(
+                staged_row<kStage>(
                     lo[r], hi[r], lane,
                     [&](int base) {
 #pragma unroll
@@ -437,7 +439,7 @@ __device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, float w
     P.acc[i] = a;
 }
 
-template <bool kBatch>
+template <bool kBatch, int kStage>
 __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     // one LDS slice per wave: staged rows (pos, ev) or SMS batches
     __shared__ float4 s_wave[kNT / 64][2 * kStage];
@@ -475,7 +477,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 const uint32_t mr = (active && !self_test)
                                         ? P.nbmask[(size_t)r * P.mask_stride + idx] : 0u;
                 const int lor = lo[r];
-                staged_row(
+                staged_row<kStage>(
                     lo[r], hi[r], lane,
                     [&](int base) {
 #pragma unroll
@@ -705,18 +707,48 @@ void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s) {
     if (blocks > 0) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kNT), 0, s, src, dst, n);
 }
 
+template <bool B, int S>
+void launch_density_t(const PhysicsArgs &p, hipStream_t s) {
+    hipLaunchKernelGGL((k_density<B, S>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+}
+
+template <bool B, int S>
+void launch_force_t(const PhysicsArgs &p, hipStream_t s) {
+    hipLaunchKernelGGL((k_force<B, S>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+}
+
+template <bool B>
+void launch_density_b(const PhysicsArgs &p, hipStream_t s) {
+    if (p.stage_depth == 64)
+        launch_density_t<B, 64>(p, s);
+    else if (p.stage_depth == 256)
+        launch_density_t<B, 256>(p, s);
+    else
+        launch_density_t<B, kStageDefault>(p, s);
+}
+
+template <bool B>
+void launch_force_b(const PhysicsArgs &p, hipStream_t s) {
+    if (p.stage_depth == 64)
+        launch_force_t<B, 64>(p, s);
+    else if (p.stage_depth == 256)
+        launch_force_t<B, 256>(p, s);
+    else
+        launch_force_t<B, kStageDefault>(p, s);
+}
+
 void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s) {
     if (batch)
-        hipLaunchKernelGGL(k_density<true>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+        launch_density_b<true>(p, s);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+        launch_density_b<false>(p, s);
 }
 
 void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s) {
     if (batch)
-        hipLaunchKernelGGL(k_force<true>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+        launch_force_b<true>(p, s);
     else
-        hipLaunchKernelGGL(k_force<false>, dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+        launch_force_b<false>(p, s);
 }
 
 void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s) {

@@ -220,6 +220,7 @@ void gsph_options_default(gsph_options *out) {
     out->blocks_per_cu = 8;
     out->pair_masks = 1;
     out->row_prune = 1;
+    out->stage_depth = 128;
 }
 
 int gsph_derive_constants(const gsph_params *p, gsph_constants *c) {

@@ -73,6 +73,7 @@ class Options(C.Structure):
     _fields_ = [
         ("split_mode", C.c_int), ("sms_min_self", C.c_int), ("sms_min_total", C.c_int),
         ("blocks_per_cu", C.c_int), ("pair_masks", C.c_int), ("row_prune", C.c_int),
+        ("stage_depth", C.c_int),
     ]
 
 
@@ -133,6 +134,9 @@ _SIGS = {
     "gsph_integrate": (_I, [_P]),
     "gsph_step": (_I, [_P, _I, _P]),
     "gsph_enable_graph": (_I, [_P, _I]),
+    "gsph_autotune": (_I, [_P, _I, _I, C.POINTER(Options), C.POINTER(C.c_float)]),
+    "gsph_set_options": (_I, [_P, C.POINTER(Options)]),
+    "gsph_get_options": (_I, [_P, C.POINTER(Options)]),
     "gsph_sync": (_I, [_P]),
     "gsph_read_state": (_I, [_P, _P, _P, _P, _P]),
     "gsph_grid_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
@@ -320,6 +324,22 @@ class System:
 
     def enable_graph(self, on=True):
         self._c(self._lib.gsph_enable_graph(self.h, 1 if on else 0))
+
+    def autotune(self, trial_steps=3, tune_split=False):
+        """Pick launch options by timing this scene (state is restored);
+        returns (options, ms per step)."""
+        o, ms = Options(), C.c_float()
+        self._c(self._lib.gsph_autotune(self.h, trial_steps, 1 if tune_split else 0,
+                                        C.byref(o), C.byref(ms)))
+        return o, ms.value
+
+    def set_options(self, opt: Options):
+        self._c(self._lib.gsph_set_options(self.h, C.byref(opt)))
+
+    def options(self) -> Options:
+        o = Options()
+        self._c(self._lib.gsph_get_options(self.h, C.byref(o)))
+        return o
 
     def sync(self):
         self._c(self._lib.gsph_sync(self.h))

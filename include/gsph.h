@@ -116,7 +116,10 @@ typedef struct gsph_stage_ms {
 /* Scheduling options.  split_mode 0 = the reference split heuristic
  * (src/io/gpu_model.cu:172-180, bit-exact task assignment); 1 = tuned
  * thresholds below (same physics, different TRA/SMS assignment).  The
- * remaining fields are launch-shape knobs chosen by gsph_autotune. */
+ * remaining fields are launch-shape knobs; gsph_autotune picks them by
+ * timing the context's own scene.  Only split_mode/thresholds change which
+ * path a particle takes (results move within FP32 rounding); the others give
+ * bit-identical results. */
 typedef struct gsph_options {
     int split_mode;
     int sms_min_self;     /* tuned: cell is TRA when n <  sms_min_self ... */
@@ -124,6 +127,7 @@ typedef struct gsph_options {
     int blocks_per_cu;    /* persistent grid size of density/force, per CU */
     int pair_masks;       /* 1: density records accepted pairs for force */
     int row_prune;        /* 1: skip rows/layers farther than h (exact superset) */
+    int stage_depth;      /* LDS staging depth of TRA neighbour rows: 64, 128, 256 */
 } gsph_options;
 
 typedef struct gsph_ctx gsph_ctx;
@@ -232,6 +236,19 @@ int gsph_integrate(gsph_ctx *ctx);
 int gsph_step(gsph_ctx *ctx, int nsteps, gsph_stage_ms *per_step_ms);
 /* Captures one full step as a hipGraph; later gsph_step calls replay it. */
 int gsph_enable_graph(gsph_ctx *ctx, int enable);
+/* Autotuner (replaces the reference's offline performance-model tables,
+ * src/io/gpu_model.cu:137-181 and its launch constants).  Times trial_steps
+ * steps of the context's own scene per candidate: persistent grid size
+ * (blocks_per_cu 4/8/16) x LDS staging depth (64/128/256) x pair masks, then,
+ * when tune_split is nonzero, the reference split rule against tuned
+ * thresholds.  The state (pos, vel, ev) is restored after every trial (the
+ * acceleration buffer keeps the last trial's values); the fastest options
+ * stay set on ctx and are returned in *chosen (best_ms: its step time). */
+int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split,
+                  gsph_options *chosen, float *best_ms);
+/* Options of a live context (validated; drops a captured graph). */
+int gsph_set_options(gsph_ctx *ctx, const gsph_options *opt);
+int gsph_get_options(gsph_ctx *ctx, gsph_options *out);
 int gsph_sync(gsph_ctx *ctx);
 /* Device -> host copies of the current state (sorted order).  NULL skips. */
 int gsph_read_state(gsph_ctx *ctx, float *pos4, float *vel3, float *ev4,

@@ -60,3 +60,22 @@ def mixed_scene(po, seed=2):
     sparse = np.zeros((800, 4), np.float32)
     sparse[:, :3] = rng.uniform([0.031] * 3, [0.329, 0.209, 0.269], (800, 3))
     return p, np.concatenate([a, blob, sparse])
+
+
+def skewed_scene(po, seed=5):
+    """Small analogue of the skewed-occupancy workload: a pool at spacing
+    h/2 plus a sphere at spacing h/4 (8x denser) above it."""
+    p = make_params(h=0.02, world=(0.36, 0.30, 0.36))
+    pool = po.scene_block([0.04, 0.04, 0.04], 28, 6, 28, 0.01, 0.01, seed)
+    s, r, c = 0.005, 0.045, np.array([0.18, 0.19, 0.18])
+    m = int(np.ceil(2 * r / s))
+    g = (np.arange(m) + 0.5) * s - r
+    xx, yy, zz = np.meshgrid(g, g, g, indexing="ij")
+    keep = xx ** 2 + yy ** 2 + zz ** 2 < r * r
+    rng = np.random.default_rng(seed)
+    pts = np.stack([xx[keep], yy[keep], zz[keep]], 1) + c
+    pts += rng.uniform(-0.01 * s, 0.01 * s, pts.shape)
+    sphere = np.zeros((len(pts), 4), np.float32)
+    sphere[:, :3] = pts
+    return p, np.concatenate([pool, sphere])
+

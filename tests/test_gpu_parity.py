@@ -13,7 +13,8 @@ Bars:
 import numpy as np
 import pytest
 
-from helpers import cfg0_params, dense_scene, make_params, mixed_scene, to_gsph_params
+from helpers import (cfg0_params, dense_scene, make_params, mixed_scene, skewed_scene,
+                     to_gsph_params)
 from tolerance import REL_TOL, step_errors
 
 pytestmark = pytest.mark.gpu
@@ -212,6 +213,65 @@ def test_pair_masks_change_nothing(gsph, po, name, opts):
         s.close()
     for k in range(4):
         assert np.array_equal(out[0][k], out[1][k])
+
+
+@pytest.mark.parametrize("name,opts", [("mixed", {}), ("dense", ALL_TRA)])
+@pytest.mark.parametrize("depth,bpc", [(64, 4), (256, 16)])
+def test_launch_shape_changes_nothing(gsph, po, name, opts, depth, bpc):
+    """Staging depth and persistent grid size leave every bit unchanged."""
+    op, pos = scene(po, name)
+    banner(f"launch shape {name} depth={depth} bpc={bpc}")
+    out = []
+    for o in ({}, {"stage_depth": depth, "blocks_per_cu": bpc}):
+        s = make_system(gsph, op, pos, **opts, **o)
+        s.step(2)
+        out.append(s.read_state())
+        s.close()
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k])
+
+
+def test_autotune_restores_state_and_keeps_bits(gsph, po):
+    op, pos = scene(po, "mixed")
+    banner("autotune mixed")
+    s = make_system(gsph, op, pos)
+    before = s.read_state()
+    opt, ms = s.autotune(trial_steps=1)
+    after = s.read_state()
+    for k in range(3):   # pos, vel, ev (acc is scratch, recomputed every step)
+        assert np.array_equal(before[k], after[k])
+    assert ms > 0
+    assert opt.stage_depth in (64, 128, 256) and opt.blocks_per_cu in (4, 8, 16)
+    assert s.options().stage_depth == opt.stage_depth
+    s.step(2)
+    tuned = s.read_state()
+    s.close()
+    ref = make_system(gsph, op, pos)
+    ref.step(2)
+    base = ref.read_state()
+    ref.close()
+    for k in range(4):
+        assert np.array_equal(tuned[k], base[k])
+
+
+def test_autotuned_split_skewed_scene(gsph, po):
+    """Skewed occupancy (pool + 8x denser sphere): the autotuner may move the
+    SMS/TRA split; one step stays within tolerance of the FP64 oracle."""
+    op, pos = skewed_scene(po)
+    n = len(pos)
+    banner(f"autotune split skewed n={n}")
+    s = make_system(gsph, op, pos)
+    opt, ms = s.autotune(trial_steps=1, tune_split=True)
+    print("chosen", opt.split_mode, opt.sms_min_self, opt.sms_min_total,
+          opt.blocks_per_cu, opt.stage_depth, opt.pair_masks, f"{ms:.3f} ms", flush=True)
+    s.step(1)
+    pos1, vel1, ev1, acc1 = s.read_state()
+    s.close()
+    ref = po.fp64_step(op, pos, np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32))
+    errs = step_errors(pos1, vel1, ev1, acc1, ref, op)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e} > {REL_TOL}"
 
 
 def test_graph_replay_matches_eager(gsph, po):
