@@ -243,11 +243,17 @@ def main():
                  for k in ("arrange", "density", "force", "integrate")}
     step_gbs = alg["step"] / (ms_per_step * 1e-3) / 1e9
 
-    kname = {"density": "k_density<false>", "force": "k_force<false>",
-             "integrate": "k_integrate<false>"}
+    def single_scene(table, stage):
+        """Entry of the single-scene instance of a stage's kernel (its name
+        starts with k_<stage><false), any staging-depth argument)."""
+        for k, v in table.items():
+            if k.startswith(f"k_{stage}<false"):
+                return v
+        return {}
+
     traffic = profile_block(TRAFFIC_PROFILE).get("traffic", {})
     valu = profile_block(VALU_PROFILE).get("counters", {})
-    dom_traffic = traffic.get(kname[dom], {}).get("bytes")
+    dom_traffic = single_scene(traffic, dom).get("bytes")
     out = {
         "metric": METRIC,
         "value": value,
@@ -268,8 +274,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": dom_traffic, "algorithmic_bytes": alg[dom], "avg_ms": dom_ms,
                      "traffic_source": TRAFFIC_PROFILE.name if dom_traffic else None,
-                     "valu_busy_pct": valu.get(kname[dom], {}).get("valu_busy_pct"),
-                     "valu_util_pct": valu.get(kname[dom], {}).get("valu_util_pct")},
+                     "valu_busy_pct": single_scene(valu, dom).get("valu_busy_pct"),
+                     "valu_util_pct": single_scene(valu, dom).get("valu_util_pct")},
         "step_roofline": {"algorithmic_bytes": alg["step"], "achieved_gbs": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "stages_ms": stage_avg,

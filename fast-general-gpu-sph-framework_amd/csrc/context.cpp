@@ -459,13 +459,13 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
 // Persistent density/force grid from blocks_per_cu: a multiple of 8 blocks
 // so that every XCD gets the same share.
 void set_grid(gsph_ctx *ctx) {
-    const int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 8;
+    const int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 16;
     ctx->grid_blocks = std::max(8, (ctx->cus * bpc + 7) / 8 * 8);
 }
 
 // Validates o; 0 selects the default of blocks_per_cu and stage_depth.
 int check_options(gsph_ctx *ctx, gsph_options &o) {
-    if (o.blocks_per_cu == 0) o.blocks_per_cu = 8;
+    if (o.blocks_per_cu == 0) o.blocks_per_cu = 16;
     if (o.stage_depth == 0) o.stage_depth = 128;
     if (o.stage_depth != 64 && o.stage_depth != 128 && o.stage_depth != 256)
         return fail(ctx, GSPH_ERR_INVALID_ARG, "stage_depth must be 64, 128 or 256");
@@ -987,7 +987,7 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
             best = o;
         }
     };
-    for (int bpc : {4, 8, 16})
+    for (int bpc : {4, 8, 16, 32})
         for (int depth : {64, 128, 256})
             for (int masks : {1, 0}) {
                 gsph_options o = start;

@@ -239,7 +239,7 @@ int gsph_enable_graph(gsph_ctx *ctx, int enable);
 /* Autotuner (replaces the reference's offline performance-model tables,
  * src/io/gpu_model.cu:137-181 and its launch constants).  Times trial_steps
  * steps of the context's own scene per candidate: persistent grid size
- * (blocks_per_cu 4/8/16) x LDS staging depth (64/128/256) x pair masks, then,
+ * (blocks_per_cu 4/8/16/32) x LDS staging depth (64/128/256) x pair masks, then,
  * when tune_split is nonzero, the reference split rule against tuned
  * thresholds.  The state (pos, vel, ev) is restored after every trial (the
  * acceleration buffer keeps the last trial's values); the fastest options
