@@ -324,7 +324,8 @@ __device__ __forceinline__ int layer_bound(const uint32_t *hash, int lo, int hi,
     return lo;
 }
 
-// Dense begin[numc+1] / count[numc] plus x-layer starts of occupied cells.
+// Dense begin[numc+1] plus x-layer starts of occupied cells (a cell's count
+// is begin[c+1] - begin[c]; no dense count array is written).
 // Block q fills cells [4096 q, 4096 q + 4096) in 4 rounds of 1024; each
 // thread owns 4 consecutive cells (16-byte stores).  The occupied cells of a
 // round are marked in a 1024-bit LDS map; a thread's cursor into the
@@ -347,20 +348,16 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
             kcur = a.chunk_k0[q];          // written by k_cells
     }
     // empty chunk (most of a sparsely filled box): begin is the next
-    // occupied cell's start, count is 0 - no map, no barriers
+    // occupied cell's start - no map, no barriers
     const int chunk_end = min(q * kFillChunk + kFillChunk, a.numc);
     if (kcur >= K || a.cells[kcur] >= chunk_end) {
         const int bg = kcur < K ? a.cstart[kcur] : a.n;
-        const int4 b4 = make_int4(bg, bg, bg, bg), z4 = make_int4(0, 0, 0, 0);
+        const int4 b4 = make_int4(bg, bg, bg, bg);
         for (int c0 = q * kFillChunk + 4 * tid; c0 < chunk_end; c0 += 4 * kNT) {
             if (c0 + 3 < a.numc) {
                 *reinterpret_cast<int4 *>(a.begin + c0) = b4;
-                *reinterpret_cast<int4 *>(a.count + c0) = z4;
             } else {
-                for (int u = 0; u < 4 && c0 + u < a.numc; ++u) {
-                    a.begin[c0 + u] = bg;
-                    a.count[c0 + u] = 0;
-                }
+                for (int u = 0; u < 4 && c0 + u < a.numc; ++u) a.begin[c0 + u] = bg;
             }
         }
         if (q == a.nchunks - 1 && tid == 0) a.begin[a.numc] = a.n;
@@ -400,14 +397,13 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
         const uint32_t nib = (word >> sh) & 15u;
         int k = kcur + (int)s_pre[w] + __popc(word & ((1u << sh) - 1u));
         const int c0 = base + 4 * tid;
-        int bg[4], cn[4];
+        int bg[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const bool occ = (nib >> u) & 1u;
             bg[u] = k < K ? a.cstart[k] : a.n;
-            cn[u] = occ ? a.cstart[k + 1] - bg[u] : 0;
             if (occ && c0 + u < a.numc) {
-                const int cnt = cn[u];
+                const int cnt = a.cstart[k + 1] - bg[u];
                 if (cnt > 65535) atomicOr(&a.dc[kCntErr], kErrCapacity);
                 const int e = bg[u] + cnt;
                 const int l1 = layer_bound(a.hash, bg[u], e, 1);
@@ -420,13 +416,9 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
         }
         if (c0 + 3 < a.numc) {
             *reinterpret_cast<int4 *>(a.begin + c0) = make_int4(bg[0], bg[1], bg[2], bg[3]);
-            *reinterpret_cast<int4 *>(a.count + c0) = make_int4(cn[0], cn[1], cn[2], cn[3]);
         } else {
             for (int u = 0; u < 4; ++u)
-                if (c0 + u < a.numc) {
-                    a.begin[c0 + u] = bg[u];
-                    a.count[c0 + u] = cn[u];
-                }
+                if (c0 + u < a.numc) a.begin[c0 + u] = bg[u];
         }
         kcur += (int)s_pre[kRound / 32];
         __syncthreads();
@@ -446,13 +438,14 @@ __device__ __forceinline__ int cell_req(const ArrangeArgs &a, int c, int n) {
     const int nc = slice * g.gz;
     const int x = local % g.gx;
     const int y = (local / g.gx) % g.gy;
+    auto count = [&](int d) { return a.begin[d + 1] - a.begin[d]; };
     int tot = n;
-    if (x > 0) tot += a.count[c - 1];
-    if (x + 1 < g.gx) tot += a.count[c + 1];
-    if (y > 0) tot += a.count[c - g.gx];
-    if (y + 1 < g.gy) tot += a.count[c + g.gx];
-    if (local >= slice) tot += a.count[c - slice];
-    if (local + slice < nc) tot += a.count[c + slice];
+    if (x > 0) tot += count(c - 1);
+    if (x + 1 < g.gx) tot += count(c + 1);
+    if (y > 0) tot += count(c - g.gx);
+    if (y + 1 < g.gy) tot += count(c + g.gx);
+    if (local >= slice) tot += count(c - slice);
+    if (local + slice < nc) tot += count(c + slice);
     const int min_self = a.split_mode ? a.sms_min_self : 15;
     const int min_tot = a.split_mode ? a.sms_min_total : 96;
     if (tot < min_tot && n < min_self) return 0;

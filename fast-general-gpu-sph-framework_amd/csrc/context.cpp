@@ -61,7 +61,7 @@ struct gsph_ctx {
     int *vals[2] = {nullptr, nullptr};
     uint32_t *tilecnt = nullptr, *digit_total = nullptr, *bcnt = nullptr, *split_sum = nullptr;
     int *cells = nullptr, *cstart = nullptr, *chunk_k0 = nullptr;
-    int *begin = nullptr, *count = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
+    int *begin = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
     uint2 *xcum = nullptr;
     int *compact = nullptr;
     uint32_t *nbmask = nullptr;   // [10][cap_n] accepted-pair masks
@@ -217,7 +217,6 @@ ArrangeArgs arrange_args(gsph_ctx *ctx) {
     a.nchunks = ctx->nchunks;
     a.chunk_k0 = ctx->chunk_k0;
     a.begin = ctx->begin;
-    a.count = ctx->count;
     a.xcum = ctx->xcum;
     a.req = ctx->req;
     a.tra_base = ctx->tra_base;
@@ -436,7 +435,6 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->cstart, N + 1);
     ALLOC(ctx->chunk_k0, nchunks + 1);
     ALLOC(ctx->begin, C + 1);
-    ALLOC(ctx->count, C);
     ALLOC(ctx->req, C);
     ALLOC(ctx->tra_base, C);
     ALLOC(ctx->sms_base, C);
@@ -604,9 +602,9 @@ namespace gsph {
 
 size_t engine_bytes(long long n, long long cells) {
     // per particle: 2 x state (44 B), acc, 2 x key/value, cell list, compact,
-    // task slot, pair masks; per cell: begin, count, req, TRA/SMS bases,
+    // task slot, pair masks; per cell: begin, req, TRA/SMS bases,
     // x-layer starts
-    return (size_t)(n * 236 + cells * 28 + (1 << 20));
+    return (size_t)(n * 236 + cells * 24 + (1 << 20));
 }
 
 int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,
@@ -1103,21 +1101,24 @@ int gsph_read_grid(gsph_ctx *ctx, int *hash, int *perm, int *begin, int *count, 
         HIP_TRY(ctx, hipMemcpyAsync(hash, ctx->keys[ctx->passes & 1], 4 * n, hipMemcpyDeviceToHost, s));
     if (n && perm)
         HIP_TRY(ctx, hipMemcpyAsync(perm, ctx->vals[ctx->passes & 1], 4 * n, hipMemcpyDeviceToHost, s));
-    if (begin) HIP_TRY(ctx, hipMemcpyAsync(begin, ctx->begin, 4 * C, hipMemcpyDeviceToHost, s));
-    std::vector<int> cnt_tmp;
-    int *cnt = count;
-    if (req && !cnt) {
-        cnt_tmp.resize(C);
-        cnt = cnt_tmp.data();
+    // counts are begin differences (no dense count array on the device)
+    std::vector<int> bg_tmp;
+    int *bg = nullptr;
+    if (begin || count || req) {
+        bg_tmp.resize(C + 1);
+        bg = bg_tmp.data();
+        HIP_TRY(ctx, hipMemcpyAsync(bg, ctx->begin, 4 * (C + 1), hipMemcpyDeviceToHost, s));
     }
-    if (cnt) HIP_TRY(ctx, hipMemcpyAsync(cnt, ctx->count, 4 * C, hipMemcpyDeviceToHost, s));
     if (req) HIP_TRY(ctx, hipMemcpyAsync(req, ctx->req, 4 * C, hipMemcpyDeviceToHost, s));
     if (n && compact)
         HIP_TRY(ctx, hipMemcpyAsync(compact, ctx->compact, 4 * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (begin) std::copy(bg, bg + C, begin);
+    if (count)
+        for (size_t c = 0; c < C; ++c) count[c] = bg[c + 1] - bg[c];
     if (req)
         for (size_t c = 0; c < C; ++c)
-            if (cnt[c] == 0) req[c] = 0;  // the sparse table is only valid in occupied cells
+            if (bg[c + 1] == bg[c]) req[c] = 0;  // the sparse table is only valid in occupied cells
     if (tasks) {
         const int T = dc[kCntT];
         const int nd = T + (T & 1);

@@ -52,8 +52,7 @@ struct ArrangeArgs {
     int *cstart;           // [n+1] first sorted particle of each occupied cell
     int nchunks;
     int *chunk_k0;         // [nchunks] first occupied cell index of each chunk
-    int *begin;            // [numc+1]
-    int *count;            // [numc]
+    int *begin;            // [numc+1]; count of cell c = begin[c+1] - begin[c]
     uint2 *xcum;           // [numc] x-layer starts (valid where count > 0)
     int *req;              // [numc] valid where count > 0
     int *tra_base;         // [numc] valid where count > 0

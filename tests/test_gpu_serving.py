@@ -57,10 +57,10 @@ def test_results_same_alone_and_in_the_mix(gsph, po):
 def test_budget_defers_and_rejects(gsph, po):
     op = make_params(h=0.02, world=(1, 1, 1))
     reqs = request_scenes(po, k=4, seed=8)
-    # the library's admission estimate: 236 B per particle, 28 B per cell, 1 MiB
+    # the library's admission estimate: 236 B per particle, 24 B per cell, 1 MiB
     def est(w, p):
         cells = np.prod([int(np.ceil(np.float32(x) / np.float32(0.02))) for x in w])
-        return len(p) * 236 + cells * 28 + (1 << 20)
+        return len(p) * 236 + cells * 24 + (1 << 20)
     sizes = [est(w, p) for (w, p, _, _) in reqs]
     need = max(sizes) + min(sizes) // 2          # never two requests at once
     srv = gsph.Server(to_gsph_params(gsph, op), hbm_budget_bytes=need, max_scenes=64)
