@@ -195,6 +195,43 @@ __device__ __forceinline__ bool in_range(float d2, PairRange pr) {
     return __float_as_uint(d2) - pr.lo <= pr.span;
 }
 
+// One SMS work item: tasks t0 = 2q and t0 + 1, one per half-wave.  When both
+// are tasks of one cell (a judgeTask isSame pair) their rows are the same
+// cell rows, and the wave stages the union of their x-layer ranges once, 64
+// candidates per batch.  The layers only one task needs lie more than 4
+// layers (h) beyond every particle of the other task, so each lane still
+// accepts exactly its own pairs, in the same index order.
+struct SmsItem {
+    bool has_task, active, shared;
+    int self, si, gidx, cx, cy, cz, gl, gr;
+};
+
+template <bool kBatch>
+__device__ __forceinline__ SmsItem sms_item(const PhysicsArgs &P, int t0, int T, int lane) {
+    SmsItem m;
+    const int task = t0 + (lane >> 5);
+    m.has_task = task < T;
+    gsph_task tk;
+    if (m.has_task) tk = P.tasks[task];
+    const int cell_end = m.has_task ? tk.cell_begin + tk.cell_particle_count : 0;
+    m.self = m.has_task ? tk.cell_begin + tk.p_offset + (lane & 31) : 0;
+    m.active = m.has_task && m.self < cell_end;
+    m.si = m.has_task ? min(m.self, cell_end - 1) : 0;
+    m.gidx = m.has_task ? scene_of_particle<kBatch>(P.scenes, P.nscenes, tk.cell_begin) : 0;
+    m.cx = m.has_task ? tk.cell_pos[0] : 0;
+    m.cy = m.has_task ? tk.cell_pos[1] : 0;
+    m.cz = m.has_task ? tk.cell_pos[2] : 0;
+    int xi = m.has_task ? tk.xxi : 0, xa = m.has_task ? tk.xxx : 0;
+    m.shared = t0 + 1 < T && P.tasks[t0].cellid == P.tasks[t0 + 1].cellid;
+    if (m.shared) {
+        xi = min((int)P.tasks[t0].xxi, (int)P.tasks[t0 + 1].xxi);
+        xa = max((int)P.tasks[t0].xxx, (int)P.tasks[t0 + 1].xxx);
+    }
+    m.gl = 4 * m.cx - 4 + xi;
+    m.gr = 4 * m.cx + 4 + xa;
+    return m;
+}
+
 // ---------------------------------------------------------------------------
 // Density.  rho = m*poly6 * sum (h^2 - r^2)^3 + self_density over
 // 1e-12 <= r^2 <= h^2; w = 1/rho into pos.w, p = ((rho/rho0)^7 - 1)*gas into
@@ -327,42 +364,35 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 if (P.use_masks) P.nbmask[(size_t)9 * P.mask_stride + idx] = long_row ? 1u : 0u;
             }
         } else {
-            const int task = ((it - tra_tiles) << 1) + (lane >> 5);
-            const int hl = lane & 31;
-            const bool has_task = task < T;
-            gsph_task tk;
-            if (has_task) tk = P.tasks[task];
-            const int cell_end = has_task ? tk.cell_begin + tk.cell_particle_count : 0;
-            const int self = has_task ? tk.cell_begin + tk.p_offset + hl : 0;
-            const bool active = has_task && self < cell_end;
-            const int si = has_task ? min(self, cell_end - 1) : 0;
-            const float4 pi = P.pos[si];
-            int gidx = has_task ? scene_of_particle<kBatch>(P.scenes, P.nscenes, tk.cell_begin) : 0;
-            const SceneGeom g = P.scenes[gidx];
-            const int cx = has_task ? tk.cell_pos[0] : 0;
-            const int cy = has_task ? tk.cell_pos[1] : 0;
-            const int cz = has_task ? tk.cell_pos[2] : 0;
-            const int gl = 4 * cx - 4 + (has_task ? tk.xxi : 0);
-            const int gr = 4 * cx + 4 + (has_task ? tk.xxx : 0);
+            const SmsItem sm = sms_item<kBatch>(P, (it - tra_tiles) << 1, T, lane);
+            const float4 pi = P.pos[sm.si];
+            const SceneGeom g = P.scenes[sm.gidx];
+            const int width = sm.shared ? 64 : 32;
+            const int sl = sm.shared ? lane : (lane & 31);
+            float4 *lds = sm.shared ? s_nb[wave] : &s_nb[wave][lane & 32];
             float s = 0.f;
-            float4 *lds = &s_nb[wave][lane & 32];
             for (int r = 0; r < 9; ++r) {
-                const int ny = cy + r % 3 - 1, nz = cz + r / 3 - 1;
+                const int ny = sm.cy + r % 3 - 1, nz = sm.cz + r / 3 - 1;
                 int lo = 0, hi = 0;
-                if (has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
-                    row_range(g, P.begin, P.xcum, cx, ny, nz, max(gl, 0),
-                              min(gr, g.gx * 4 - 1), lo, hi);
+                if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
+                    row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
+                              min(sm.gr, g.gx * 4 - 1), lo, hi);
                 float rs = 0.f;
-                for (int b0 = lo; b0 < hi; b0 += 32) {
-                    const int cnt = min(32, hi - b0);
-                    if (hl < cnt) lds[hl] = P.pos[b0 + hl];
+                for (int b0 = lo; b0 < hi; b0 += width) {
+                    const int cnt = min(width, hi - b0);
+                    if (sl < cnt) lds[sl] = P.pos[b0 + sl];
                     wave_sync_lds();
-                    for (int k = 0; k < cnt; ++k) rs += density_term(pi, lds[k], h2, pr);
+                    int k = 0;
+                    for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) rs += density_term(pi, lds[k + u], h2, pr);
+                    }
+                    for (; k < cnt; ++k) rs += density_term(pi, lds[k], h2, pr);
                     wave_sync_lds();
                 }
                 s += rs;
             }
-            if (active) density_store(P, self, s);
+            if (sm.active) density_store(P, sm.self, s);
         }
     }
 }
@@ -529,52 +559,51 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             }
             if (active) force_store(P, i, pi.w, A);
         } else {
-            const int task = ((it - tra_tiles) << 1) + (lane >> 5);
-            const int hl = lane & 31;
-            const bool has_task = task < T;
-            gsph_task tk;
-            if (has_task) tk = P.tasks[task];
-            const int cell_end = has_task ? tk.cell_begin + tk.cell_particle_count : 0;
-            const int self = has_task ? tk.cell_begin + tk.p_offset + hl : 0;
-            const bool active = has_task && self < cell_end;
-            const int si = has_task ? min(self, cell_end - 1) : 0;
-            const float4 pi = P.pos[si];
-            const float4 ei = P.ev[si];
-            int gidx = has_task ? scene_of_particle<kBatch>(P.scenes, P.nscenes, tk.cell_begin) : 0;
-            const SceneGeom g = P.scenes[gidx];
-            const int cx = has_task ? tk.cell_pos[0] : 0;
-            const int cy = has_task ? tk.cell_pos[1] : 0;
-            const int cz = has_task ? tk.cell_pos[2] : 0;
-            const int gl = 4 * cx - 4 + (has_task ? tk.xxi : 0);
-            const int gr = 4 * cx + 4 + (has_task ? tk.xxx : 0);
+            const SmsItem sm = sms_item<kBatch>(P, (it - tra_tiles) << 1, T, lane);
+            const float4 pi = P.pos[sm.si];
+            const float4 ei = P.ev[sm.si];
+            const SceneGeom g = P.scenes[sm.gidx];
+            const int width = sm.shared ? 64 : 32;
+            const int sl = sm.shared ? lane : (lane & 31);
+            float4 *lp = sm.shared ? s_wave[wave] : s_wave[wave] + (lane & 32);
+            float4 *le = lp + 64;
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            float4 *lp = s_wave[wave] + (lane & 32);
-            float4 *le = s_wave[wave] + 64 + (lane & 32);
             for (int r = 0; r < 9; ++r) {
-                const int ny = cy + r % 3 - 1, nz = cz + r / 3 - 1;
+                const int ny = sm.cy + r % 3 - 1, nz = sm.cz + r / 3 - 1;
                 int lo = 0, hi = 0;
-                if (has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
-                    row_range(g, P.begin, P.xcum, cx, ny, nz, max(gl, 0),
-                              min(gr, g.gx * 4 - 1), lo, hi);
-                for (int b0 = lo; b0 < hi; b0 += 32) {
-                    const int cnt = min(32, hi - b0);
-                    if (hl < cnt) {
-                        lp[hl] = P.pos[b0 + hl];
-                        le[hl] = P.ev[b0 + hl];
+                if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
+                    row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
+                              min(sm.gr, g.gx * 4 - 1), lo, hi);
+                for (int b0 = lo; b0 < hi; b0 += width) {
+                    const int cnt = min(width, hi - b0);
+                    if (sl < cnt) {
+                        lp[sl] = P.pos[b0 + sl];
+                        le[sl] = P.ev[b0 + sl];
                     }
                     wave_sync_lds();
-                    uint32_t mask = 0;
-                    for (int k = 0; k < cnt; ++k)
-                        mask |= (in_range(dist2(pi, lp[k]), pr) ? 1u : 0u) << k;
-                    while (mask) {
-                        const int k = __builtin_ctz(mask);
-                        mask &= mask - 1u;
-                        force_pair(pi, ei, lp[k], le[k], P.c, h2d, A);
+                    // 32 candidates at a time: accept mask, then the pair body
+                    for (int k0 = 0; k0 < cnt; k0 += 32) {
+                        const int n = min(32, cnt - k0);
+                        const float4 *q = lp + k0;
+                        uint32_t mask = 0;   // bits enter at the bottom, reversed below
+                        int k = 0;
+                        for (; k + 4 <= n; k += 4) {
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
+                        }
+                        for (; k < n; ++k) mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
+                        mask = __builtin_bitreverse32(mask) >> (32 - n);
+                        while (mask) {
+                            const int kk = k0 + __builtin_ctz(mask);
+                            mask &= mask - 1u;
+                            force_pair(pi, ei, lp[kk], le[kk], P.c, h2d, A);
+                        }
                     }
                     wave_sync_lds();
                 }
             }
-            if (active) force_store(P, self, pi.w, A);
+            if (sm.active) force_store(P, sm.self, pi.w, A);
         }
     }
 }
