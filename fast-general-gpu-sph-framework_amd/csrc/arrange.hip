@@ -654,6 +654,7 @@ __global__ __launch_bounds__(kNT) void k_compact(ArrangeArgs a) {
             a.tasks[T] = pad;
         }
         a.dc[kCntMiddle] = a.dc[kCntSms] > 0 ? total_tra : -1;
+        a.dc[kCntSmsPool] = 0;   // density allocates this step's SMS masks afresh
     }
 }
 
@@ -667,6 +668,7 @@ __global__ void k_empty_grid(ArrangeArgs a) {
         a.dc[kCntTra] = 0;
         a.dc[kCntSms] = 0;
         a.dc[kCntMiddle] = -1;
+        a.dc[kCntSmsPool] = 0;
     }
 }
 

@@ -28,6 +28,7 @@ enum CounterSlot {
     kCntSms = 3,        // SMS particles
     kCntMiddle = 4,     // reference d_middle_value_ (-1 without SMS)
     kCntErr = 5,
+    kCntSmsPool = 6,    // rows of the SMS mask pool taken this step
     kCntSlots = 8,
 };
 

@@ -65,6 +65,9 @@ struct gsph_ctx {
     uint2 *xcum = nullptr;
     int *compact = nullptr;
     uint32_t *nbmask = nullptr;   // [10][cap_n] accepted-pair masks
+    uint32_t *sms_pool = nullptr; // SMS accepted-pair masks, rows of 64 words
+    int sms_pool_rows = 0;
+    int *sms_off = nullptr;       // first pool row of each SMS item
     gsph_task *tasks = nullptr;
     int *dc = nullptr;
     SceneGeom *scenes_d = nullptr;
@@ -256,6 +259,10 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.mask_stride = ctx->cap_n;
     p.use_masks = 0;
     p.stage_depth = ctx->opt.stage_depth;
+    p.sms_pool = ctx->sms_pool;
+    p.sms_pool_rows = ctx->sms_pool_rows;
+    p.sms_off = ctx->sms_off;
+    p.dcw = ctx->dc;
     return p;
 }
 
@@ -441,6 +448,15 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->xcum, C);
     ALLOC(ctx->compact, N);
     ALLOC(ctx->nbmask, 10 * N);
+    // SMS masks: about one 32-candidate sub-batch row per 8 bytes of budget
+    // per particle (rows hold 64 lanes x 4 B)
+    {
+        const size_t rows = ctx->opt.sms_mask_bytes > 0
+                                ? N * (size_t)ctx->opt.sms_mask_bytes / 256 : 0;
+        ctx->sms_pool_rows = (int)std::min<size_t>(rows, (size_t)INT32_MAX / 2);
+        ALLOC(ctx->sms_pool, (size_t)ctx->sms_pool_rows * 64);
+        ALLOC(ctx->sms_off, task_cap / 2 + 2);
+    }
     ALLOC(ctx->tasks, task_cap);
     ALLOC(ctx->dc, kCntSlots);
     ALLOC(ctx->scenes_d, ctx->cap_s);
@@ -471,6 +487,8 @@ int check_options(gsph_ctx *ctx, gsph_options &o) {
         return fail(ctx, GSPH_ERR_INVALID_ARG, "blocks_per_cu must be in [1, 32]");
     if (o.split_mode != 0 && o.split_mode != 1)
         return fail(ctx, GSPH_ERR_INVALID_ARG, "split_mode must be 0 or 1");
+    if (o.sms_mask_bytes < 0 || o.sms_mask_bytes > 65536)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "sms_mask_bytes must be in [0, 65536]");
     return GSPH_OK;
 }
 
@@ -602,9 +620,11 @@ namespace gsph {
 
 size_t engine_bytes(long long n, long long cells) {
     // per particle: 2 x state (44 B), acc, 2 x key/value, cell list, compact,
-    // task slot, pair masks; per cell: begin, req, TRA/SMS bases,
-    // x-layer starts
-    return (size_t)(n * 236 + cells * 24 + (1 << 20));
+    // task slot, pair masks (+ the SMS mask budget); per cell: begin, req,
+    // TRA/SMS bases, x-layer starts
+    gsph_options o;
+    gsph_options_default(&o);
+    return (size_t)(n * (236 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
 }
 
 int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,

@@ -92,6 +92,14 @@ struct PhysicsArgs {
     int mask_stride;
     int use_masks;
     int stage_depth;       // LDS staging depth of TRA rows: 64, 128 or 256
+    // accepted-pair masks of SMS items: a pool of rows of 64 words (one per
+    // lane); density takes W rows per item (W = 32-candidate sub-batches of
+    // its longest half) with an atomic on dc[kCntSmsPool] and records the
+    // first row in sms_off[item] (-1: pool full, force tests that item)
+    uint32_t *sms_pool;
+    int sms_pool_rows;
+    int *sms_off;          // [task_cap / 2 + 1]
+    int *dcw;              // writable alias of dc
 };
 
 // arrange.hip

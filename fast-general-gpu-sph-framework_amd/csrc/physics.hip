@@ -364,30 +364,63 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 if (P.use_masks) P.nbmask[(size_t)9 * P.mask_stride + idx] = long_row ? 1u : 0u;
             }
         } else {
-            const SmsItem sm = sms_item<kBatch>(P, (it - tra_tiles) << 1, T, lane);
+            const int item = it - tra_tiles;
+            const SmsItem sm = sms_item<kBatch>(P, item << 1, T, lane);
             const float4 pi = P.pos[sm.si];
             const SceneGeom g = P.scenes[sm.gidx];
             const int width = sm.shared ? 64 : 32;
             const int sl = sm.shared ? lane : (lane & 31);
             float4 *lds = sm.shared ? s_nb[wave] : &s_nb[wave][lane & 32];
-            float s = 0.f;
+            int lo[9], hi[9], subs = 0;
+#pragma unroll
             for (int r = 0; r < 9; ++r) {
                 const int ny = sm.cy + r % 3 - 1, nz = sm.cz + r / 3 - 1;
-                int lo = 0, hi = 0;
+                lo[r] = hi[r] = 0;
                 if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
                     row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
-                              min(sm.gr, g.gx * 4 - 1), lo, hi);
+                              min(sm.gr, g.gx * 4 - 1), lo[r], hi[r]);
+                subs += (hi[r] - lo[r] + 31) >> 5;
+            }
+            // pool rows of this item: the longer half's 32-candidate sub-batches
+            const int rows = max(__builtin_amdgcn_readlane(subs, 0),
+                                 __builtin_amdgcn_readlane(subs, 32));
+            int base = -1;
+            if (P.use_masks) {
+                int b = -1;
+                if (lane == 0) {
+                    // no increment once the pool is full, so the counter
+                    // cannot run past it by more than one row set per wave
+                    const int limit = P.sms_pool_rows - rows;
+                    if (rows > 0 && limit >= 0 &&
+                        __hip_atomic_load(&P.dcw[kCntSmsPool], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT) <= limit) {
+                        b = atomicAdd(&P.dcw[kCntSmsPool], rows);
+                        if (b < 0 || b > limit) b = -1;
+                    }
+                    P.sms_off[item] = b;
+                }
+                base = __builtin_amdgcn_readfirstlane(b);
+            }
+            float s = 0.f;
+            int w = 0;
+            for (int r = 0; r < 9; ++r) {
                 float rs = 0.f;
-                for (int b0 = lo; b0 < hi; b0 += width) {
-                    const int cnt = min(width, hi - b0);
+                for (int b0 = lo[r]; b0 < hi[r]; b0 += width) {
+                    const int cnt = min(width, hi[r] - b0);
                     if (sl < cnt) lds[sl] = P.pos[b0 + sl];
                     wave_sync_lds();
-                    int k = 0;
-                    for (; k + 4 <= cnt; k += 4) {
+                    for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
+                        const int n = min(32, cnt - k0);
+                        const float4 *q = lds + k0;
+                        uint32_t mk = 0;
+                        int k = 0;
+                        for (; k + 4 <= n; k += 4) {
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) rs += density_term(pi, lds[k + u], h2, pr);
+                            for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
+                        }
+                        for (; k < n; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
+                        if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_mask(mk, n);
                     }
-                    for (; k < cnt; ++k) rs += density_term(pi, lds[k], h2, pr);
                     wave_sync_lds();
                 }
                 s += rs;
@@ -559,9 +592,13 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             }
             if (active) force_store(P, i, pi.w, A);
         } else {
-            const SmsItem sm = sms_item<kBatch>(P, (it - tra_tiles) << 1, T, lane);
+            const int item = it - tra_tiles;
+            const SmsItem sm = sms_item<kBatch>(P, item << 1, T, lane);
             const float4 pi = P.pos[sm.si];
             const float4 ei = P.ev[sm.si];
+            // first pool row of the density pass's masks (-1: test here)
+            const int base = P.use_masks ? P.sms_off[item] : -1;
+            int w = 0;
             const SceneGeom g = P.scenes[sm.gidx];
             const int width = sm.shared ? 64 : 32;
             const int sl = sm.shared ? lane : (lane & 31);
@@ -582,18 +619,24 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                     }
                     wave_sync_lds();
                     // 32 candidates at a time: accept mask, then the pair body
-                    for (int k0 = 0; k0 < cnt; k0 += 32) {
+                    for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                         const int n = min(32, cnt - k0);
                         const float4 *q = lp + k0;
-                        uint32_t mask = 0;   // bits enter at the bottom, reversed below
-                        int k = 0;
-                        for (; k + 4 <= n; k += 4) {
+                        uint32_t mask = 0;
+                        if (base >= 0) {
+                            mask = P.sms_pool[(size_t)(base + w) * 64 + lane];
+                        } else {
+                            // bits enter at the bottom, reversed below
+                            int k = 0;
+                            for (; k + 4 <= n; k += 4) {
 #pragma unroll
-                            for (int u = 0; u < 4; ++u)
-                                mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
+                                for (int u = 0; u < 4; ++u)
+                                    mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
+                            }
+                            for (; k < n; ++k)
+                                mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
+                            mask = __builtin_bitreverse32(mask) >> (32 - n);
                         }
-                        for (; k < n; ++k) mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
-                        mask = __builtin_bitreverse32(mask) >> (32 - n);
                         while (mask) {
                             const int kk = k0 + __builtin_ctz(mask);
                             mask &= mask - 1u;

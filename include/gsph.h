@@ -128,6 +128,8 @@ typedef struct gsph_options {
     int pair_masks;       /* 1: density records accepted pairs for force */
     int row_prune;        /* 1: skip rows/layers farther than h (exact superset) */
     int stage_depth;      /* LDS staging depth of TRA neighbour rows: 64, 128, 256 */
+    int sms_mask_bytes;   /* SMS accepted-pair mask pool, bytes per particle of
+                             capacity (set at creation; 0 = none) */
 } gsph_options;
 
 typedef struct gsph_ctx gsph_ctx;

@@ -198,11 +198,14 @@ def test_determinism_and_prune_equivalence(gsph, po):
 ALL_TRA = {"split_mode": 1, "sms_min_self": 1 << 20, "sms_min_total": 1 << 20}
 
 
-@pytest.mark.parametrize("name,opts", [("mixed", {}), ("dense", ALL_TRA)])
+@pytest.mark.parametrize("name,opts", [("mixed", {}), ("dense", ALL_TRA), ("dense", {}),
+                                       ("odd_tasks", {}), ("dense", {"sms_mask_bytes": 16})])
 def test_pair_masks_change_nothing(gsph, po, name, opts):
     """Force walking density's accepted-pair masks gives the same bits as force
     testing every candidate itself (dense all-TRA rows exceed one mask word,
-    so those particles take the self-testing path inside the same waves)."""
+    so those particles take the self-testing path inside the same waves).
+    SMS items record their masks in a pool; a pool too small for all items
+    (16 B per particle) makes the rest test their candidates in force."""
     op, pos = scene(po, name)
     banner(f"pair masks {name} {opts}")
     out = []
