@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -77,6 +78,8 @@ struct gsph_ctx {
     int cur = 0;              // buffer holding the current state (0 = A, 1 = B)
     bool grid_valid = false;  // arrange outputs match the state in B
     bool masks_valid = false; // nbmask holds the pair masks of this arrangement
+    long long steps = 0;      // completed steps (integrations)
+    std::vector<std::array<float, 3>> worlds;  // box of each scene (plain and batch contexts)
     // graph replay of one full step
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -325,6 +328,7 @@ int do_integrate(gsph_ctx *ctx) {
     ctx->cur = 0;
     ctx->grid_valid = false;
     ctx->masks_valid = false;
+    ++ctx->steps;
     return GSPH_OK;
 }
 
@@ -757,7 +761,9 @@ int gsph_create(const gsph_params *p, const gsph_options *opt, int n, const floa
     int rc = gsph_derive_constants(p, &c);
     if (rc) return rc;
     std::vector<SceneGeom> sc(1, geom_for(c, p->world_size, 0, n));
-    return build_context(p, opt, sc, n, pos4, vel3, false, out);
+    rc = build_context(p, opt, sc, n, pos4, vel3, false, out);
+    if (!rc) (*out)->worlds.push_back({p->world_size[0], p->world_size[1], p->world_size[2]});
+    return rc;
 }
 
 int gsph_batch_create(const gsph_params *p, const gsph_options *opt, int nscenes,
@@ -776,7 +782,45 @@ int gsph_batch_create(const gsph_params *p, const gsph_options *opt, int nscenes
         total += counts[s];
         if (total > 0x7fffffffLL) return fail(nullptr, GSPH_ERR_CAPACITY, "too many particles");
     }
-    return build_context(p, opt, sc, (int)total, pos4, vel3, true, out);
+    rc = build_context(p, opt, sc, (int)total, pos4, vel3, true, out);
+    if (!rc)
+        for (int s = 0; s < nscenes; ++s)
+            (*out)->worlds.push_back({world3[3 * s], world3[3 * s + 1], world3[3 * s + 2]});
+    return rc;
+}
+
+int gsph_get_params(const gsph_ctx *ctx, gsph_params *out) {
+    if (!ctx || !out) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    *out = ctx->params;
+    return GSPH_OK;
+}
+
+int gsph_scene_world(const gsph_ctx *ctx, int scene, float world[3]) {
+    if (!ctx || !world) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    if (ctx->worlds.size() != ctx->scenes.size())
+        return fail(nullptr, GSPH_ERR_INVALID_ARG, "scene boxes are not kept by this context");
+    if (scene < 0 || scene >= (int)ctx->worlds.size())
+        return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad scene index");
+    for (int a = 0; a < 3; ++a) world[a] = ctx->worlds[scene][a];
+    return GSPH_OK;
+}
+
+int gsph_is_batched(const gsph_ctx *ctx, int *batched) {
+    if (!ctx || !batched) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    *batched = ctx->batch ? 1 : 0;
+    return GSPH_OK;
+}
+
+int gsph_step_count(const gsph_ctx *ctx, long long *steps) {
+    if (!ctx || !steps) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null argument");
+    *steps = ctx->steps;
+    return GSPH_OK;
+}
+
+int gsph_set_step_count(gsph_ctx *ctx, long long steps) {
+    if (!ctx || steps < 0) return fail(ctx, GSPH_ERR_INVALID_ARG, "bad arguments");
+    ctx->steps = steps;
+    return GSPH_OK;
 }
 
 int gsph_scene_count(const gsph_ctx *ctx, int *nscenes) {
@@ -913,6 +957,7 @@ int gsph_step(gsph_ctx *ctx, int nsteps, gsph_stage_ms *per_step_ms) {
                 if (rc) return rc;
             }
             HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+            ++ctx->steps;
             ctx->cur = 0;
             ctx->grid_valid = false;
             ctx->masks_valid = false;
@@ -971,6 +1016,7 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
     if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
     if (ctx->cur != 0) return fail(ctx, GSPH_ERR_INVALID_ARG, "autotune expects the integrated state");
     const gsph_options start = ctx->opt;
+    const long long steps0 = ctx->steps;
     if (ctx->n == 0) {
         if (chosen) *chosen = start;
         if (best_ms) *best_ms = 0.f;
@@ -1032,6 +1078,7 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
     (void)hipFree(vel0);
     (void)hipFree(ev0);
     ctx->opt = rc ? start : best;
+    ctx->steps = steps0;   // trials do not count as steps of the simulation
     set_grid(ctx);
     if (ctx->graph_exec) {   // captured with the old launch shape
         (void)hipGraphExecDestroy(ctx->graph_exec);

@@ -123,6 +123,15 @@ _SIGS = {
                                C.POINTER(_P)]),
     "gsph_scene_count": (_I, [_P, C.POINTER(_I)]),
     "gsph_scene_range": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
+    "gsph_scene_world": (_I, [_P, _I, C.POINTER(C.c_float)]),
+    "gsph_get_params": (_I, [_P, C.POINTER(Params)]),
+    "gsph_is_batched": (_I, [_P, C.POINTER(_I)]),
+    "gsph_step_count": (_I, [_P, C.POINTER(C.c_longlong)]),
+    "gsph_set_step_count": (_I, [_P, C.c_longlong]),
+    "gsph_snapshot_save": (_I, [_P, C.c_char_p]),
+    "gsph_snapshot_check": (_I, [C.c_char_p, C.POINTER(C.c_longlong), C.POINTER(_I),
+                                 C.POINTER(C.c_longlong)]),
+    "gsph_snapshot_load": (_I, [C.c_char_p, C.POINTER(Options), C.POINTER(_P)]),
     "gsph_create": (_I, [C.POINTER(Params), C.POINTER(Options), _I, _P, _P, C.POINTER(_P)]),
     "gsph_destroy": (_I, [_P]),
     "gsph_num_particles": (_I, [_P, C.POINTER(_I)]),
@@ -256,9 +265,17 @@ class System:
     """One context: a plain scene (gsph_create) or a batch (gsph_batch_create)."""
 
     def __init__(self, params: Params, pos4, vel3=None, options: Options | None = None,
-                 *, batch_worlds=None, batch_counts=None):
+                 *, batch_worlds=None, batch_counts=None, _handle=None):
         lib = load()
         self._lib = lib
+        if _handle is not None:             # adopt a context made by the library
+            self.h = _handle
+            n = C.c_int()
+            self._c(lib.gsph_num_particles(self.h, C.byref(n)))
+            self.n = n.value
+            self.params = Params()
+            self._c(lib.gsph_get_params(self.h, C.byref(self.params)))
+            return
         pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
         vel = None if vel3 is None else np.ascontiguousarray(vel3, np.float32).reshape(-1, 3)
         opt = options if options is not None else options_default()
@@ -275,6 +292,15 @@ class System:
         self.h = h
         self.n = pos4.shape[0]
         self.params = params
+
+    def save(self, path):
+        """Checkpoint: parameters, options, scene layout, step count, state."""
+        self._c(self._lib.gsph_snapshot_save(self.h, os.fsencode(path)))
+
+    def step_count(self) -> int:
+        v = C.c_longlong()
+        self._c(self._lib.gsph_step_count(self.h, C.byref(v)))
+        return v.value
 
     def close(self):
         if self.h:
@@ -474,3 +500,20 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def snapshot_check(path):
+    """Validates a snapshot file without the GPU: (n, scenes, steps)."""
+    lib = load()
+    n, ns, st = C.c_longlong(), C.c_int(), C.c_longlong()
+    _check(lib.gsph_snapshot_check(os.fsencode(path), C.byref(n), C.byref(ns), C.byref(st)))
+    return n.value, ns.value, st.value
+
+
+def load_snapshot(path, options: Options | None = None) -> System:
+    """A context restored from a snapshot (continues bit-exactly)."""
+    lib = load()
+    h = C.c_void_p()
+    _check(lib.gsph_snapshot_load(os.fsencode(path), None if options is None else C.byref(options),
+                                  C.byref(h)))
+    return System(None, None, _handle=h)

@@ -175,6 +175,27 @@ int gsph_batch_create(const gsph_params *p, const gsph_options *opt, int nscenes
                       const float *vel3, gsph_ctx **out);
 int gsph_scene_count(const gsph_ctx *ctx, int *nscenes);
 int gsph_scene_range(const gsph_ctx *ctx, int scene, int *first, int *count);
+/* Box of a scene as given at creation (plain and batch contexts). */
+int gsph_scene_world(const gsph_ctx *ctx, int scene, float world[3]);
+/* Parameters a context was created with; whether it is a batch context. */
+int gsph_get_params(const gsph_ctx *ctx, gsph_params *out);
+int gsph_is_batched(const gsph_ctx *ctx, int *batched);
+/* Completed steps (integrations) of a context; settable for restarts. */
+int gsph_step_count(const gsph_ctx *ctx, long long *steps);
+int gsph_set_step_count(gsph_ctx *ctx, long long steps);
+
+/* Checkpoint / resume.  No counterpart in the reference, whose state only
+ * moves between device and host buffers (ParticleBufferObject::transfer,
+ * src/particle/particle_buffer.cpp:369-381).  A snapshot file holds the
+ * parameters, options, scene boxes and counts, the step count and the state
+ * (pos4, vel3, ev4) in the context's current slot order, with an FNV-1a 64
+ * checksum; a context loaded from it continues bit-exactly.  The file is
+ * written under path + ".tmp" and renamed.  gsph_snapshot_check validates a
+ * file without touching the GPU.  opt (nullable) overrides the stored
+ * options on load. */
+int gsph_snapshot_save(gsph_ctx *ctx, const char *path);
+int gsph_snapshot_check(const char *path, long long *n, int *nscenes, long long *steps);
+int gsph_snapshot_load(const char *path, const gsph_options *opt, gsph_ctx **out);
 
 /* Serving form --------------------------------------------------------------- */
 /* No counterpart in the reference.  A continuous-batching request loop:

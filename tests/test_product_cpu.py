@@ -99,3 +99,59 @@ def test_create_without_device_fails_cleanly(gsph):
     with pytest.raises(gsph.GsphError) as e:
         gsph.System(p, pos)
     assert e.value.code == -5
+
+
+def _fnv1a(data: bytes, h=14695981039346656037):
+    for b in data:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _write_snapshot(gsph, path, pos4, vel3, ev4, worlds, counts, steps, batched):
+    """The documented snapshot layout, written independently of the library."""
+    import ctypes as C
+    import struct
+    p = gsph.params_default()
+    o = gsph.options_default()
+    body = bytes(p) + bytes(o)
+    for w, c in zip(worlds, counts):
+        body += struct.pack("<3fi", *w, c)
+    body += pos4.astype("<f4").tobytes() + vel3.astype("<f4").tobytes() + ev4.astype("<f4").tobytes()
+    head = b"GSPHSNAP" + struct.pack("<6I4Q", 1, 64, C.sizeof(gsph.Params), C.sizeof(gsph.Options),
+                                     1 if batched else 0, len(counts), len(pos4), steps,
+                                     _fnv1a(body), 0)
+    assert len(head) == 64
+    path.write_bytes(head + body)
+
+
+def test_snapshot_check_validates_without_gpu(gsph, tmp_path):
+    rng = np.random.default_rng(3)
+    n = 37
+    pos = rng.uniform(0.1, 0.5, (n, 4)).astype(np.float32)
+    vel = rng.normal(size=(n, 3)).astype(np.float32)
+    ev = rng.normal(size=(n, 4)).astype(np.float32)
+    good = tmp_path / "s.gsnap"
+    _write_snapshot(gsph, good, pos, vel, ev, [[0.6] * 3, [0.4] * 3], [20, 17], 123, True)
+    assert gsph.snapshot_check(good) == (37, 2, 123)
+    # every kind of damage is refused before any device work
+    with pytest.raises(gsph.GsphError) as e:
+        gsph.snapshot_check(tmp_path / "missing.gsnap")
+    assert e.value.code == -7
+    data = bytearray(good.read_bytes())
+    cases = {
+        "magic": lambda d: d.__setitem__(slice(0, 1), b"X"),
+        "truncated": lambda d: d.__delitem__(slice(len(d) - 5, len(d))),
+        "payload": lambda d: d.__setitem__(len(d) - 3, d[len(d) - 3] ^ 0x10),
+        "counts": None,
+    }
+    for name, damage in cases.items():
+        bad = tmp_path / f"bad_{name}.gsnap"
+        if damage is None:     # scene counts that do not add up to n
+            _write_snapshot(gsph, bad, pos, vel, ev, [[0.6] * 3], [36], 0, False)
+        else:
+            d = bytearray(data)
+            damage(d)
+            bad.write_bytes(bytes(d))
+        with pytest.raises(gsph.GsphError) as e:
+            gsph.snapshot_check(bad)
+        assert e.value.code == -7, name
