@@ -236,20 +236,15 @@ __device__ __forceinline__ SmsItem sms_item(const PhysicsArgs &P, int t0, int T,
 // Density.  rho = m*poly6 * sum (h^2 - r^2)^3 + self_density over
 // 1e-12 <= r^2 <= h^2; w = 1/rho into pos.w, p = ((rho/rho0)^7 - 1)*gas into
 // ev.w (density_kernels.cu:353-358, 536-544).
-__device__ __forceinline__ float density_term(const float4 &a, const float4 &b, float h2,
-                                              PairRange pr) {
-    const float d2 = dist2(a, b);
-    const float t = h2 - d2;
-    return in_range(d2, pr) ? t * t * t : 0.f;
-}
-
-// density_term plus the acceptance bit shifted into mk
+// One candidate of the density sum: rs += (h^2 - r^2)^3 if the pair is
+// accepted (a rejected one adds an exact zero), and its acceptance bit is
+// shifted into mk.
 __device__ __forceinline__ void density_pair(const float4 &a, const float4 &b, float h2,
                                              PairRange pr, float &rs, uint32_t &mk) {
     const float d2 = dist2(a, b);
-    const float t = h2 - d2;
     const bool ok = in_range(d2, pr);
-    rs += ok ? t * t * t : 0.f;
+    const float t = ok ? h2 - d2 : 0.f;
+    rs = fmaf(t * t, t, rs);
     mk += mk + (ok ? 1u : 0u);
 }
 
@@ -405,10 +400,14 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             int w = 0;
             for (int r = 0; r < 9; ++r) {
                 float rs = 0.f;
+                // the next batch's candidate is loaded while this one computes
+                float4 nx = pi;
+                if (sl < min(width, hi[r] - lo[r])) nx = P.pos[lo[r] + sl];
                 for (int b0 = lo[r]; b0 < hi[r]; b0 += width) {
                     const int cnt = min(width, hi[r] - b0);
-                    if (sl < cnt) lds[sl] = P.pos[b0 + sl];
+                    if (sl < cnt) lds[sl] = nx;
                     wave_sync_lds();
+                    if (sl < min(width, hi[r] - b0 - width)) nx = P.pos[b0 + width + sl];
                     for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                         const int n = min(32, cnt - k0);
                         const float4 *q = lds + k0;
@@ -611,20 +610,39 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
                     row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
                               min(sm.gr, g.gx * 4 - 1), lo, hi);
+                // the next batch's candidate (and its masks) are loaded while
+                // this batch computes
+                float4 np = pi, ne = ei;
+                uint32_t nm0 = 0, nm1 = 0;
+                auto fetch = [&](int b, int wb) {
+                    const int c = min(width, hi - b);
+                    if (sl < c) {
+                        np = P.pos[b + sl];
+                        ne = P.ev[b + sl];
+                    }
+                    if (base >= 0 && c > 0) {
+                        const size_t row0 = (size_t)(base + wb) * 64 + lane;
+                        nm0 = P.sms_pool[row0];
+                        if (c > 32) nm1 = P.sms_pool[row0 + 64];
+                    }
+                };
+                fetch(lo, w);
                 for (int b0 = lo; b0 < hi; b0 += width) {
                     const int cnt = min(width, hi - b0);
                     if (sl < cnt) {
-                        lp[sl] = P.pos[b0 + sl];
-                        le[sl] = P.ev[b0 + sl];
+                        lp[sl] = np;
+                        le[sl] = ne;
                     }
+                    const uint32_t cm0 = nm0, cm1 = nm1;
                     wave_sync_lds();
+                    fetch(b0 + width, w + ((cnt + 31) >> 5));
                     // 32 candidates at a time: accept mask, then the pair body
                     for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                         const int n = min(32, cnt - k0);
                         const float4 *q = lp + k0;
                         uint32_t mask = 0;
                         if (base >= 0) {
-                            mask = P.sms_pool[(size_t)(base + w) * 64 + lane];
+                            mask = k0 == 0 ? cm0 : cm1;
                         } else {
                             // bits enter at the bottom, reversed below
                             int k = 0;
