@@ -324,8 +324,8 @@ __device__ __forceinline__ int layer_bound(const uint32_t *hash, int lo, int hi,
     return lo;
 }
 
-// Dense begin[numc+1] plus x-layer starts of occupied cells (a cell's count
-// is begin[c+1] - begin[c]; no dense count array is written).
+// Dense begin[numc+1] (a cell's count is begin[c+1] - begin[c]; no dense
+// count array is written).
 // Block q fills cells [4096 q, 4096 q + 4096) in 4 rounds of 1024; each
 // thread owns 4 consecutive cells (16-byte stores).  The occupied cells of a
 // round are marked in a 1024-bit LDS map; a thread's cursor into the
@@ -405,12 +405,6 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
             if (occ && c0 + u < a.numc) {
                 const int cnt = a.cstart[k + 1] - bg[u];
                 if (cnt > 65535) atomicOr(&a.dc[kCntErr], kErrCapacity);
-                const int e = bg[u] + cnt;
-                const int l1 = layer_bound(a.hash, bg[u], e, 1);
-                const int l2 = layer_bound(a.hash, l1, e, 2);
-                const int l3 = layer_bound(a.hash, l2, e, 3);
-                a.xcum[c0 + u] = make_uint2((uint32_t)(l1 - bg[u]) | ((uint32_t)(l2 - bg[u]) << 16),
-                                            (uint32_t)(l3 - bg[u]));
             }
             k += occ ? 1 : 0;
         }
@@ -453,6 +447,27 @@ __device__ __forceinline__ int cell_req(const ArrangeArgs &a, int c, int n) {
     return (n + 31) >> 5;
 }
 
+// x-layer starts of an occupied cell: the number of its particles (sorted
+// keys hash[pb, pb + n)) whose micro x layer is < 1, < 2, < 3, packed
+// 16 + 16 | 16 bits.  Small cells are counted in one pass over their keys
+// (independent loads), large ones by binary search.
+__device__ __forceinline__ uint2 x_layer_starts(const uint32_t *hash, int pb, int n) {
+    int l1 = 0, l2 = 0, l3 = 0;
+    if (n <= 32) {
+        for (int j = 0; j < n; ++j) {
+            const uint32_t L = (hash[pb + j] >> 4) & 3u;
+            l1 += L < 1u;
+            l2 += L < 2u;
+            l3 += L < 3u;
+        }
+    } else {
+        l1 = layer_bound(hash, pb, pb + n, 1) - pb;
+        l2 = layer_bound(hash, pb + l1, pb + n, 2) - pb;
+        l3 = layer_bound(hash, pb + l2, pb + n, 3) - pb;
+    }
+    return make_uint2((uint32_t)l1 | ((uint32_t)l2 << 16), (uint32_t)l3);
+}
+
 template <bool kBatch>
 __global__ __launch_bounds__(kNT) void k_split_count(ArrangeArgs a) {
     __shared__ uint32_t s_tmp[8];
@@ -463,7 +478,9 @@ __global__ __launch_bounds__(kNT) void k_split_count(ArrangeArgs a) {
         const int k = blockIdx.x * kSplitTile + s * kNT + threadIdx.x;
         if (k >= K) break;
         const int c = a.cells[k];
-        const int n = a.cstart[k + 1] - a.cstart[k];
+        const int pb = a.cstart[k];
+        const int n = a.cstart[k + 1] - pb;
+        a.xcum[c] = x_layer_starts(a.hash, pb, n);
         const int r = cell_req<kBatch>(a, c, n);
         const int nsms = min(n, r * kSmsTaskParticles);
         st += (uint32_t)(n - nsms);
