@@ -48,18 +48,19 @@ __device__ __forceinline__ int scene_of_particle(const SceneGeom *sc, int ns,
     return lo;
 }
 
-// #particles of cell c whose micro x layer is < L (L in 0..4).
-__device__ __forceinline__ int layers_below(const int *begin, const uint2 *xcum,
-                                            int c, int L) {
-    if (L <= 0) return 0;
-    const int cnt = begin[c + 1] - begin[c];
-    if (L >= 4 || cnt == 0) return L >= 4 ? cnt : 0;
-    const uint2 x = xcum[c];
-    return L == 1 ? (int)(x.x & 0xffffu) : L == 2 ? (int)(x.x >> 16) : (int)x.y;
+// #particles of a cell (begin b0, next cell's begin b1, x-layer starts x)
+// whose micro x layer is < L (L in 0..4).  x is only meaningful for an
+// occupied cell; selects instead of branches, so the loads feeding it can
+// all be in flight at once.
+__device__ __forceinline__ int layer_count(int b0, int b1, uint2 x, int L) {
+    const int cnt = b1 - b0;
+    const int v = L == 1 ? (int)(x.x & 0xffffu) : L == 2 ? (int)(x.x >> 16) : (int)x.y;
+    return L <= 0 ? 0 : L >= 4 ? cnt : cnt == 0 ? 0 : v;
 }
 
-// Sorted range of row (ny, nz) for a particle of cell column cx whose
-// candidates span micro-x layers [gl, gr] (global micro-x indices).
+// Sorted range of row (ny, nz) for candidates spanning micro-x layers
+// [gl, gr] (global micro-x indices).  Six independent loads: begin and the
+// next begin of the left and right cells, and their x-layer starts.
 __device__ __forceinline__ void row_range(const SceneGeom &g, const int *begin,
                                           const uint2 *xcum, int cx, int ny,
                                           int nz, int gl, int gr, int &lo,
@@ -68,9 +69,11 @@ __device__ __forceinline__ void row_range(const SceneGeom &g, const int *begin,
     const int lcx = max(gl >> 2, 0);
     const int rcx = min(gr >> 2, g.gx - 1);
     const int lc = rowc + lcx, rc = rowc + rcx;
-    lo = begin[lc] + ((gl >> 2) == lcx ? layers_below(begin, xcum, lc, gl & 3) : 0);
-    hi = begin[rc] + ((gr >> 2) == rcx ? layers_below(begin, xcum, rc, (gr & 3) + 1)
-                                       : begin[rc + 1] - begin[rc]);
+    const int bl0 = begin[lc], bl1 = begin[lc + 1];
+    const int br0 = begin[rc], br1 = begin[rc + 1];
+    const uint2 xl = xcum[lc], xr = xcum[rc];
+    lo = bl0 + ((gl >> 2) == lcx ? layer_count(bl0, bl1, xl, gl & 3) : 0);
+    hi = br0 + ((gr >> 2) == rcx ? layer_count(br0, br1, xr, (gr & 3) + 1) : br1 - br0);
     (void)cx;
 }
 
