@@ -300,11 +300,58 @@ __device__ __forceinline__ void staged_row(int lo, int hi, int lane, Stage &&sta
     (void)lane;
 }
 
+// All 9 rows of a TRA tile walked as one sequence of staging rounds (rows in
+// order; within a row as in staged_row), software-pipelined by the callers:
+// while round t is visited, the loads of round t + 1 (possibly the next
+// row's first) are in flight.  Rows empty for every lane are skipped.  The
+// row bounds come from LDS (rlo/rhi: row r of lane l at r * 64 + l), so the
+// wave-uniform row index never indexes a register array.
+//
+// Advances (r, cur) to the next round; cur = kNone when all rows are done.
+template <int kStage>
+__device__ __forceinline__ void next_round(int &r, int &cur, int &lo_r, int &hi_r,
+                                           const int *rlo, const int *rhi, int lane) {
+    int nxt = kNone;
+    if (r >= 0) nxt = wave_min(hi_r > cur + kStage ? max(lo_r, cur + kStage) : kNone);
+    while (nxt == kNone && r < 8) {
+        ++r;
+        lo_r = rlo[r * 64 + lane];
+        hi_r = rhi[r * 64 + lane];
+        nxt = wave_min(lo_r < hi_r ? lo_r : kNone);
+    }
+    cur = nxt;
+}
+
+// One staging round in registers: entries lane + 64 u, u < kStage / 64.
+template <int kStage>
+__device__ __forceinline__ void load_round(const float4 *src, int base, int n, int lane,
+                                           float4 &v0, float4 &v1, float4 &v2, float4 &v3) {
+    const int k = base + lane;
+    if (k < n) v0 = src[k];
+    if (kStage > 64 && k + 64 < n) v1 = src[k + 64];
+    if (kStage > 128 && k + 128 < n) v2 = src[k + 128];
+    if (kStage > 128 && k + 192 < n) v3 = src[k + 192];
+}
+
+template <int kStage>
+__device__ __forceinline__ void store_round(float4 *dst, int lane, const float4 &v0,
+                                            const float4 &v1, const float4 &v2,
+                                            const float4 &v3) {
+    dst[lane] = v0;
+    if (kStage > 64) dst[lane + 64] = v1;
+    if (kStage > 128) {
+        dst[lane + 128] = v2;
+        dst[lane + 192] = v3;
+    }
+}
+
 template <bool kBatch, int kStage>
 __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     __shared__ float4 s_nb[kNT / 64][64];
     __shared__ float4 s_stage[kNT / 64][kStage];
+    __shared__ int s_rows[kNT / 64][2][9 * 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int *rlo = s_rows[wave][0], *rhi = s_rows[wave][1];
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
     const int tra_tiles = (tra + 63) >> 6;
@@ -323,40 +370,58 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             const float4 pi = P.pos[i];
             int lo[9], hi[9];
             tra_rows<kBatch>(P, pi, active, i, lo, hi);
-            float s = 0.f;
-            bool long_row = false;
+            // per-row partial sums (rejected candidates add exact zeros, so
+            // pruning cannot change the result) and per-row accept masks
+#pragma unroll
             for (int r = 0; r < 9; ++r) {
-                // per-row partial sum; rejected candidates add exact zeros, so
-                // pruning cannot change the result
-                float rs = 0.f;
-                uint32_t mk = 0;
-                staged_row<kStage>(
-                    lo[r], hi[r], lane,
-                    [&](int base) {
-#pragma unroll
-                        for (int k = lane; k < kStage; k += 64)
-                            if (base + k < P.n) stg[k] = P.pos[base + k];
-                    },
-                    [&](int jb, int je, int base) {
-                        const int cnt = je - jb;
-                        if (cnt <= 0) return;
-                        const float4 *q = stg + (jb - base);
-                        int k = 0;
-                        for (; k + 4 <= cnt; k += 4) {
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
-                        }
-                        for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
-                    });
-                s += rs;
-                if (P.use_masks) {
-                    const int cnt = hi[r] - lo[r];
-                    long_row |= cnt > kMaskRowMax;
-                    if (active)
-                        P.nbmask[(size_t)r * P.mask_stride + idx] =
-                            cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
-                }
+                rlo[r * 64 + lane] = lo[r];
+                rhi[r * 64 + lane] = hi[r];
             }
+            // per-row partial sums (rejected candidates add exact zeros, so
+            // pruning cannot change the result) and per-row accept masks
+            float s = 0.f, rs = 0.f;
+            uint32_t mk = 0;
+            bool long_row = false;
+            int done = 0;   // rows [0, done) are finished
+            auto finish_rows = [&](int upto) __attribute__((always_inline)) {
+                for (; done < upto; ++done) {
+                    s += rs;
+                    rs = 0.f;
+                    if (P.use_masks) {
+                        const int cnt = rhi[done * 64 + lane] - rlo[done * 64 + lane];
+                        long_row |= cnt > kMaskRowMax;
+                        if (active)
+                            P.nbmask[(size_t)done * P.mask_stride + idx] =
+                                cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
+                    }
+                    mk = 0;
+                }
+            };
+            float4 v0 = pi, v1 = pi, v2 = pi, v3 = pi;
+            int r = -1, cur = kNone, lo_r = 0, hi_r = 0;
+            next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
+            if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
+            while (cur != kNone) {
+                const int rr = r, base = cur, lo_c = lo_r, hi_c = hi_r;
+                store_round<kStage>(stg, lane, v0, v1, v2, v3);
+                wave_sync_lds();
+                next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
+                if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
+                finish_rows(rr);
+                const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
+                const int cnt = je - jb;
+                if (cnt > 0) {
+                    const float4 *q = stg + (jb - base);
+                    int k = 0;
+                    for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
+                    }
+                    for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
+                }
+                wave_sync_lds();
+            }
+            finish_rows(9);
             if (active) {
                 density_store(P, i, s);
                 if (P.use_masks) P.nbmask[(size_t)9 * P.mask_stride + idx] = long_row ? 1u : 0u;
@@ -508,7 +573,9 @@ template <bool kBatch, int kStage>
 __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     // one LDS slice per wave: staged rows (pos, ev) or SMS batches
     __shared__ float4 s_wave[kNT / 64][2 * kStage];
+    __shared__ int s_rows[kNT / 64][2][9 * 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int *rlo = s_rows[wave][0], *rhi = s_rows[wave][1];
     float4 *sp = s_wave[wave];
     float4 *se = sp + kStage;
     const int tra = P.dc[kCntTra];
@@ -538,59 +605,73 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             // test the staged candidates first.  Same pairs, same order.
             const bool self_test =
                 !P.use_masks || (active && P.nbmask[(size_t)9 * P.mask_stride + idx] != 0);
+            const bool use_mask = P.use_masks && active && !self_test;
+#pragma unroll
             for (int r = 0; r < 9; ++r) {
-                const uint32_t mr = (active && !self_test)
-                                        ? P.nbmask[(size_t)r * P.mask_stride + idx] : 0u;
-                const int lor = lo[r];
-                staged_row<kStage>(
-                    lo[r], hi[r], lane,
-                    [&](int base) {
+                rlo[r * 64 + lane] = lo[r];
+                rhi[r * 64 + lane] = hi[r];
+            }
+            float4 p0 = pi, p1 = pi, p2 = pi, p3 = pi, e0 = ei, e1 = ei, e2 = ei, e3 = ei;
+            uint32_t m_ld = 0, m_st = 0;   // row mask of the loaded / the staged round
+            int r_ld = -1;
+            int r = -1, cur = kNone, lo_r = 0, hi_r = 0;
+            auto load = [&]() __attribute__((always_inline)) {
+                load_round<kStage>(P.pos, cur, P.n, lane, p0, p1, p2, p3);
+                load_round<kStage>(P.ev, cur, P.n, lane, e0, e1, e2, e3);
+                if (r != r_ld) {
+                    r_ld = r;
+                    m_ld = use_mask ? P.nbmask[(size_t)r * P.mask_stride + idx] : 0u;
+                }
+            };
+            next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
+            if (cur != kNone) load();
+            while (cur != kNone) {
+                const int base = cur, lo_c = lo_r, hi_c = hi_r;
+                store_round<kStage>(sp, lane, p0, p1, p2, p3);
+                store_round<kStage>(se, lane, e0, e1, e2, e3);
+                m_st = m_ld;
+                wave_sync_lds();
+                next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
+                if (cur != kNone) load();
+                const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
+                const float4 *q = sp + (jb - base);
+                const float4 *qe = se + (jb - base);
+                if (!self_test) {
+                    const int cnt = je - jb;   // <= 32, and jb - lo < 32
+                    uint32_t mm = 0;
+                    if (cnt > 0) {
+                        mm = m_st >> (jb - lo_c);
+                        if (cnt < 32) mm &= (1u << cnt) - 1u;
+                    }
+                    while (mm) {
+                        const int kk = __builtin_ctz(mm);
+                        mm &= mm - 1u;
+                        force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
+                    }
+                } else {
+                    // phase 1 tests up to 32 candidates into a mask, phase 2
+                    // runs the pair body over the accepted ones (in order)
+                    for (int b0 = 0; b0 < je - jb; b0 += 32) {
+                        const int cnt = min(32, je - jb - b0);
+                        // bits enter at the bottom (reversed order, undone below)
+                        uint32_t mask = 0;
+                        int k = 0;
+                        for (; k + 4 <= cnt; k += 4) {
 #pragma unroll
-                        for (int k = lane; k < kStage; k += 64)
-                            if (base + k < P.n) {
-                                sp[k] = P.pos[base + k];
-                                se[k] = P.ev[base + k];
-                            }
-                    },
-                    [&](int jb, int je, int base) {
-                        const float4 *q = sp + (jb - base);
-                        const float4 *qe = se + (jb - base);
-                        if (!self_test) {
-                            const int cnt = je - jb;   // <= 32, and jb - lo < 32
-                            uint32_t mm = 0;
-                            if (cnt > 0) {
-                                mm = mr >> (jb - lor);
-                                if (cnt < 32) mm &= (1u << cnt) - 1u;
-                            }
-                            while (mm) {
-                                const int kk = __builtin_ctz(mm);
-                                mm &= mm - 1u;
-                                force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
-                            }
-                            return;
+                            for (int u = 0; u < 4; ++u)
+                                mask += mask + (in_range(dist2(pi, q[b0 + k + u]), pr) ? 1u : 0u);
                         }
-                        // phase 1 tests up to 32 candidates into a mask, phase 2
-                        // runs the pair body over the accepted ones (in order)
-                        for (int b0 = 0; b0 < je - jb; b0 += 32) {
-                            const int cnt = min(32, je - jb - b0);
-                            // bits enter at the bottom (reversed order, undone below)
-                            uint32_t mask = 0;
-                            int k = 0;
-                            for (; k + 4 <= cnt; k += 4) {
-#pragma unroll
-                                for (int u = 0; u < 4; ++u)
-                                    mask += mask + (in_range(dist2(pi, q[b0 + k + u]), pr) ? 1u : 0u);
-                            }
-                            for (; k < cnt; ++k)
-                                mask += mask + (in_range(dist2(pi, q[b0 + k]), pr) ? 1u : 0u);
-                            mask = __builtin_bitreverse32(mask) >> (32 - cnt);
-                            while (mask) {
-                                const int kk = b0 + __builtin_ctz(mask);
-                                mask &= mask - 1u;
-                                force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
-                            }
+                        for (; k < cnt; ++k)
+                            mask += mask + (in_range(dist2(pi, q[b0 + k]), pr) ? 1u : 0u);
+                        mask = __builtin_bitreverse32(mask) >> (32 - cnt);
+                        while (mask) {
+                            const int kk = b0 + __builtin_ctz(mask);
+                            mask &= mask - 1u;
+                            force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
                         }
-                    });
+                    }
+                }
+                wave_sync_lds();
             }
             if (active) force_store(P, i, pi.w, A);
         } else {
@@ -617,7 +698,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 // this batch computes
                 float4 np = pi, ne = ei;
                 uint32_t nm0 = 0, nm1 = 0;
-                auto fetch = [&](int b, int wb) {
+                auto fetch = [&](int b, int wb) __attribute__((always_inline)) {
                     const int c = min(width, hi - b);
                     if (sl < c) {
                         np = P.pos[b + sl];
