@@ -85,7 +85,7 @@ def batch_scenes(gsph, count=256, seed=20261206):
     return worlds, blocks
 
 
-def batched_leg(gsph, steps, warmup, count=256, loop_steps=5):
+def batched_leg(gsph, steps, warmup, count=256, loop_steps=5, no_autotune=False):
     import numpy as np
 
     params, _ = gsph.scene_config(0)
@@ -93,7 +93,7 @@ def batched_leg(gsph, steps, warmup, count=256, loop_steps=5):
     counts = [len(b) for b in blocks]
     n = int(sum(counts))
     b = gsph.System(params, np.concatenate(blocks), batch_worlds=worlds, batch_counts=counts)
-    tuned, _ = b.autotune(trial_steps=2)
+    tuned = b.options() if no_autotune else b.autotune(trial_steps=2)[0]
     b.enable_graph(True)
     b.step(warmup)
     b.sync()
@@ -180,6 +180,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--legs", default="plain,batched,serving")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default options (profiling runs: no trial launches)")
     ap.add_argument("--probe-copy", type=float, default=0.0,
                     help="only run a plain copy kernel over this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -204,8 +206,11 @@ def main():
     ncell = c.grid_size[0] * c.grid_size[1] * c.grid_size[2]
     print(f"[bench] workload {args.config}: n={n} cells={ncell}", file=sys.stderr, flush=True)
     s = gsph.System(params, pos)
-    tuned, tuned_ms = s.autotune(trial_steps=3)   # launch shape only: same bits
-    print(f"[bench] autotune: {opt_dict(tuned)} ({tuned_ms:.3f} ms/step)", file=sys.stderr,
+    if args.no_autotune:
+        tuned, tuned_ms = s.options(), None
+    else:
+        tuned, tuned_ms = s.autotune(trial_steps=3)   # launch shape only: same bits
+    print(f"[bench] options: {opt_dict(tuned)} (autotune {tuned_ms} ms/step)", file=sys.stderr,
           flush=True)
     s.step(args.warmup)
     s.sync()
@@ -289,7 +294,8 @@ def main():
     legs = args.legs.split(",")
     if "batched" in legs:
         print("[bench] batched leg", file=sys.stderr, flush=True)
-        out["batched"] = batched_leg(gsph, max(args.steps // 2, 5), 3)
+        out["batched"] = batched_leg(gsph, max(args.steps // 2, 5), 3,
+                                     no_autotune=args.no_autotune)
         print(f"[bench] batched {out['batched']['speedup_vs_plain_loop']:.2f}x plain loop",
               file=sys.stderr, flush=True)
     if "serving" in legs:

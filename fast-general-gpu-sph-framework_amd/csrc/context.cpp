@@ -477,13 +477,13 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
 // Persistent density/force grid from blocks_per_cu: a multiple of 8 blocks
 // so that every XCD gets the same share.
 void set_grid(gsph_ctx *ctx) {
-    const int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 16;
+    const int bpc = ctx->opt.blocks_per_cu > 0 ? ctx->opt.blocks_per_cu : 32;
     ctx->grid_blocks = std::max(8, (ctx->cus * bpc + 7) / 8 * 8);
 }
 
 // Validates o; 0 selects the default of blocks_per_cu and stage_depth.
 int check_options(gsph_ctx *ctx, gsph_options &o) {
-    if (o.blocks_per_cu == 0) o.blocks_per_cu = 16;
+    if (o.blocks_per_cu == 0) o.blocks_per_cu = 32;
     if (o.stage_depth == 0) o.stage_depth = 128;
     if (o.stage_depth != 64 && o.stage_depth != 128 && o.stage_depth != 256)
         return fail(ctx, GSPH_ERR_INVALID_ARG, "stage_depth must be 64, 128 or 256");

@@ -217,7 +217,7 @@ void gsph_options_default(gsph_options *out) {
     out->split_mode = 0;
     out->sms_min_self = 15;
     out->sms_min_total = 96;
-    out->blocks_per_cu = 16;
+    out->blocks_per_cu = 32;
     out->pair_masks = 1;
     out->row_prune = 1;
     out->stage_depth = 128;

@@ -219,7 +219,7 @@ def test_pair_masks_change_nothing(gsph, po, name, opts):
 
 
 @pytest.mark.parametrize("name,opts", [("mixed", {}), ("dense", ALL_TRA)])
-@pytest.mark.parametrize("depth,bpc", [(64, 4), (256, 32)])
+@pytest.mark.parametrize("depth,bpc", [(64, 4), (256, 8)])
 def test_launch_shape_changes_nothing(gsph, po, name, opts, depth, bpc):
     """Staging depth and persistent grid size leave every bit unchanged."""
     op, pos = scene(po, name)
