@@ -1,11 +1,15 @@
 """Benchmark of the SPH step on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cpu-steps S]
+                [--legs plain,batched,serving,reference] [--no-autotune]
 
 Workload: the 158^3 = 3,944,312-particle column in a 6.32 m cube (317^3
 cells), h = 0.02, dt = 1e-4, FP32 (synthetic workload 1).  A step is one
 pass of the hot path (arrange, density, force, integrate) over that state,
-resident in HBM.  Prints one JSON line.
+resident in HBM.  Prints one JSON line; beside the plain step it carries the
+batched (256 scenes) and serving (Poisson requests) legs, the reference's
+own default scene timed against its published frame time, and the CPU
+baseline.
 """
 import argparse
 import json
@@ -138,6 +142,40 @@ def batched_leg(gsph, steps, warmup, count=256, loop_steps=5, no_autotune=False)
     }
 
 
+# The reference's own benchmark: assets/scene_default.json (3,944,312
+# particles, h = 0.03, ~296 per cell, SMS path), mean frame 21.833 ms on an
+# RTX 5090 (README.md:212, agent_docs/optimization_report.md:194).
+REFERENCE_SCENE_MS = 21.833
+
+
+def reference_scene_leg(gsph, steps=20, warmup=3, no_autotune=False):
+    path = ROOT / "fast-general-gpu-sph-framework_amd" / "assets" / "scene_default.json"
+    params, pos = gsph.scene_json(path)
+    s = gsph.System(params, pos)
+    tuned = s.options() if no_autotune else s.autotune(trial_steps=2)[0]
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    st = s.step(5, timings=True)
+    info = s.grid_info()
+    s.close()
+    n = pos.shape[0]
+    return {
+        "workload": "the reference's default scene (assets/scene_default.json): 3,944,312 "
+                    "particles, h=0.03, ~296 per cell (SMS path)",
+        "ms_per_step": ms, "particle_steps_per_s": n * 1e3 / ms,
+        "stages_ms": {k: sum(x[k] for x in st) / len(st) for k in st[0]},
+        "sms_tasks": info["tasks"],
+        "reference_published_ms": REFERENCE_SCENE_MS,
+        "reference_published_hw": "RTX 5090 (README.md:212)",
+        "speedup_vs_reference_published": REFERENCE_SCENE_MS / ms,
+        "autotuned_options": opt_dict(tuned),
+    }
+
+
 def serving_leg(gsph, capacity_ps, requests=96, seed=20261207, budget=200e9):
     """Workload 4: Poisson arrivals, N log-uniform in [8192, 1048576], steps
     uniform in 50..500, 200 GB budget; the arrival rate puts the offered load
@@ -179,7 +217,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--legs", default="plain,batched,serving")
+    ap.add_argument("--legs", default="plain,batched,serving,reference")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default options (profiling runs: no trial launches)")
     ap.add_argument("--probe-copy", type=float, default=0.0,
@@ -302,6 +340,11 @@ def main():
         print("[bench] serving leg", file=sys.stderr, flush=True)
         cap = out.get("batched", {}).get("particle_steps_per_s", value)
         out["serving"] = serving_leg(gsph, cap)
+    if "reference" in legs:
+        print("[bench] reference scene leg", file=sys.stderr, flush=True)
+        out["reference_scene"] = reference_scene_leg(gsph, no_autotune=args.no_autotune)
+        print(f"[bench] reference scene {out['reference_scene']['ms_per_step']:.2f} ms/step",
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         print(f"[bench] cpu baseline {args.cpu_steps} steps on {threads} threads",
