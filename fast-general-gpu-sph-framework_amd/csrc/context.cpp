@@ -65,7 +65,7 @@ struct gsph_ctx {
     int *begin = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
     uint2 *xcum = nullptr;
     int *compact = nullptr;
-    uint32_t *nbmask = nullptr;   // [10][cap_n] accepted-pair masks
+    uint32_t *nbmask = nullptr;   // [19][cap_n] accepted-pair masks and row plans
     uint32_t *sms_pool = nullptr; // SMS accepted-pair masks, rows of 64 words
     int sms_pool_rows = 0;
     int *sms_off = nullptr;       // first pool row of each SMS item
@@ -451,7 +451,7 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->sms_base, C);
     ALLOC(ctx->xcum, C);
     ALLOC(ctx->compact, N);
-    ALLOC(ctx->nbmask, 10 * N);
+    ALLOC(ctx->nbmask, (size_t)kMaskWords * N);
     // SMS masks: about one 32-candidate sub-batch row per 8 bytes of budget
     // per particle (rows hold 64 lanes x 4 B)
     {
@@ -628,7 +628,7 @@ size_t engine_bytes(long long n, long long cells) {
     // TRA/SMS bases, x-layer starts
     gsph_options o;
     gsph_options_default(&o);
-    return (size_t)(n * (236 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
+    return (size_t)(n * (272 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
 }
 
 int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,

@@ -19,6 +19,7 @@ constexpr int kSortTile = 4096;     // keys per radix tile (256 threads x 16)
 constexpr int kCellTile = 4096;     // sorted particles per cell-table tile
 constexpr int kFillChunk = 4096;    // cells per fill block
 constexpr int kSplitTile = 1024;    // occupied cells per split tile
+constexpr int kMaskWords = 19;       // accepted-pair mask + row plan words per TRA particle
 
 // Everything the arrange kernels touch.  Device pointers.
 struct ArrangeArgs {
@@ -86,8 +87,9 @@ struct PhysicsArgs {
     const int *dc;
     int grid_blocks;       // persistent grid of density / force
     int row_prune;
-    // accepted-pair masks of TRA particles: 10 words per entry of compact,
-    // word-major with stride mask_stride; written by density, read by force
+    // accepted-pair masks and row plans of TRA particles: kMaskWords words per
+    // entry of compact, word-major with stride mask_stride; written by
+    // density, read by force
     uint32_t *nbmask;
     int mask_stride;
     int use_masks;

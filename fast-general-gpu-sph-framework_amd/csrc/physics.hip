@@ -184,10 +184,16 @@ __device__ __forceinline__ float dist2(const float4 &a, const float4 &b) {
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
-// Accepted-pair masks of TRA particles: word r < 9 of entry idx holds bit k
-// for candidate lo_r + k of row r (rows of at most 32 candidates); word 9 is
-// nonzero when some row is longer, and force then tests that particle itself.
+// Accepted-pair masks and row plan of TRA particles (entry idx of compact,
+// word-major with stride mask_stride).  Word r < 9 holds bit k for candidate
+// lo_r + k of row r (rows of at most 32 candidates); word 9 has bit 0 set
+// when some row is longer (force then tests that particle itself) and bit 1
+// when words 10..18 hold the row plan, so that force does not plan the rows
+// again: word 10 + r = (lo_r - i) as 24-bit two's complement | length << 24,
+// i the particle's sorted index (stored only when every row fits; an empty
+// row is stored as 0).  kMaskWords words per entry in all.
 constexpr int kMaskRowMax = 32;
+constexpr uint32_t kFlagLongRow = 1u, kFlagPlan = 2u;
 
 __device__ __forceinline__ uint32_t finish_mask(uint32_t shifted, int cnt) {
     // bits entered at the bottom in candidate order; reverse into bit k
@@ -424,7 +430,18 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             finish_rows(9);
             if (active) {
                 density_store(P, i, s);
-                if (P.use_masks) P.nbmask[(size_t)9 * P.mask_stride + idx] = long_row ? 1u : 0u;
+                if (P.use_masks) {
+                    bool fits = true;
+#pragma unroll
+                    for (int r = 0; r < 9; ++r) {
+                        const int len = hi[r] - lo[r], off = len > 0 ? lo[r] - i : 0;
+                        fits &= len <= 255 && off >= -(1 << 23) && off < (1 << 23);
+                        P.nbmask[(size_t)(10 + r) * P.mask_stride + idx] =
+                            ((uint32_t)off & 0xffffffu) | ((uint32_t)len << 24);
+                    }
+                    P.nbmask[(size_t)9 * P.mask_stride + idx] =
+                        (long_row ? kFlagLongRow : 0u) | (fits ? kFlagPlan : 0u);
+                }
             }
         } else {
             const int item = it - tra_tiles;
@@ -593,8 +610,19 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             const int i = P.compact[active ? idx : tra - 1];
             const float4 pi = P.pos[i];
             const float4 ei = P.ev[i];
+            // the density pass's row plan when it stored one, else plan here
+            const uint32_t flag = P.use_masks && active ? P.nbmask[(size_t)9 * P.mask_stride + idx] : 0u;
             int lo[9], hi[9];
-            tra_rows<kBatch>(P, pi, active, i, lo, hi);
+            if (flag & kFlagPlan) {
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    const uint32_t w = P.nbmask[(size_t)(10 + r) * P.mask_stride + idx];
+                    lo[r] = i + ((int)(w << 8) >> 8);
+                    hi[r] = lo[r] + (int)(w >> 24);
+                }
+            } else {
+                tra_rows<kBatch>(P, pi, active, i, lo, hi);
+            }
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
             // lanes whose density pass recorded masks walk their accepted
             // pairs in one flat loop (rows in order, candidates in order);
@@ -603,8 +631,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             // recorded the row's accepted-pair mask run the pair body over
             // it, the others (masks off, or a row longer than one mask word)
             // test the staged candidates first.  Same pairs, same order.
-            const bool self_test =
-                !P.use_masks || (active && P.nbmask[(size_t)9 * P.mask_stride + idx] != 0);
+            const bool self_test = !P.use_masks || (active && (flag & kFlagLongRow) != 0);
             const bool use_mask = P.use_masks && active && !self_test;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
