@@ -25,6 +25,7 @@ namespace {
 
 constexpr int kNT = 256;          // threads of the tile kernels (4 waves)
 constexpr int kItems = 16;        // items per thread: 4096-key tiles
+constexpr int kCompactItems = 4;  // particles per thread of k_compact
 
 template <bool kBatch>
 __device__ __forceinline__ int scene_of_particle(const SceneGeom *sc, int ns,
@@ -85,10 +86,7 @@ __global__ __launch_bounds__(kNT) void k_hash(ArrangeArgs a) {
             }
             a.keys[0][i] = key;
         }
-        const uint32_t d = valid ? (key & 255u) : 256u;
-        const uint64_t peers = match9(d);
-        if (valid && (peers & lanemask_lt()) == 0)
-            s_cnt[wave][d] += (uint32_t)__popcll(peers);
+        if (valid) atomicAdd(&s_cnt[wave][key & 255u], 1u);
     }
     __syncthreads();
     const uint32_t tot = s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
@@ -101,15 +99,19 @@ __global__ __launch_bounds__(kNT) void k_upsweep(ArrangeArgs a, int pass) {
     const uint32_t *keys = a.keys[pass & 1];
     const int shift = 8 * pass;
     for (int k = tid; k < 4 * 256; k += kNT) (&s_cnt[0][0])[k] = 0;
-    __syncthreads();
     const int base = blockIdx.x * kSortTile + wave * 1024;
+    uint32_t kv[kItems];
+#pragma unroll
+    for (int s = 0; s < kItems; ++s) {
+        const int i = base + s * 64 + lane;
+        kv[s] = i < a.n ? keys[i] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
     for (int s = 0; s < kItems; ++s) {
         const int i = base + s * 64 + lane;
         const bool valid = i < a.n;
-        const uint32_t d = valid ? ((keys[i] >> shift) & 255u) : 256u;
-        const uint64_t peers = match9(d);
-        if (valid && (peers & lanemask_lt()) == 0)
-            s_cnt[wave][d] += (uint32_t)__popcll(peers);
+        if (valid) atomicAdd(&s_cnt[wave][(kv[s] >> shift) & 255u], 1u);
     }
     __syncthreads();
     const uint32_t tot = s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
@@ -144,15 +146,21 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
     __shared__ uint32_t s_local[256];     // start of the digit run inside the tile
     __shared__ uint32_t s_tmp[8];
     __shared__ uint32_t s_key[kSortTile];
-    __shared__ int s_val[kSortTile];
+    __shared__ __align__(8) int s_val[kSortTile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = blockIdx.x;
+    // lane masks of the digits while the tile is ranked (in s_val's space,
+    // which is free until the reorder): 256 words per wave
+    unsigned long long *s_peer = reinterpret_cast<unsigned long long *>(s_val);
     const int shift = 8 * pass;
     const uint32_t *kin = a.keys[pass & 1];
     const int *vin = a.vals[pass & 1];
     uint32_t *kout = a.keys[(pass + 1) & 1];
     int *vout = a.vals[(pass + 1) & 1];
-    for (int k = tid; k < 4 * 256; k += kNT) (&s_w[0][0])[k] = 0;
+    for (int k = tid; k < 4 * 256; k += kNT) {
+        (&s_w[0][0])[k] = 0;
+        s_peer[k] = 0ull;
+    }
     {
         uint32_t tot;
         const uint32_t ex = block_excl_scan<kNT>(a.digit_total[tid], s_tmp, tot);
@@ -179,13 +187,23 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
     for (int s = 0; s < kItems; ++s) {
         const int i = base + s * 64 + lane;
         const bool valid = i < a.n;
-        const uint32_t d = valid ? ((key[s] >> shift) & 255u) : 256u;
-        const uint64_t peers = match9(d);
+        const uint32_t d = (key[s] >> shift) & 255u;
+        // lanes of the wave with the same digit: each sets its bit in the
+        // digit's word, then reads the word back (one wave's LDS operations
+        // complete in order, so the read sees every lane's bit); the first
+        // of them clears it again for the next slot
+        unsigned long long *pm = s_peer + wave * 256 + d;
+        if (valid) atomicOr(pm, 1ull << lane);
+        const unsigned long long peers =
+            valid ? __hip_atomic_load(pm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) : 0ull;
         const uint32_t lt = (uint32_t)__popcll(peers & lanemask_lt());
         uint32_t old = 0;
         if (valid) old = s_w[wave][d];
         rank[s] = old + lt;
-        if (valid && lt == 0) s_w[wave][d] = old + (uint32_t)__popcll(peers);
+        if (valid && lt == 0) {
+            s_w[wave][d] = old + (uint32_t)__popcll(peers);
+            __hip_atomic_store(pm, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
     }
     __syncthreads();
     uint32_t cnt_d;
@@ -229,17 +247,47 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
 __global__ __launch_bounds__(kNT) void k_gather(ArrangeArgs a) {
     __shared__ uint32_t s_tmp[8];
     const int tid = threadIdx.x;
+    constexpr int kSlots = kCellTile / kNT;
+    const int t0 = blockIdx.x * kCellTile + tid;
+    int src[kSlots];
     uint32_t nb = 0;
-    for (int s = 0; s < kCellTile / kNT; ++s) {
-        const int i = blockIdx.x * kCellTile + s * kNT + tid;
-        if (i >= a.n) break;
-        const int src = a.perm[i];
-        a.pos_b[i] = a.pos_a[src];
-        a.vel_b[i] = a.vel_a[src];
-        a.ev_b[i] = a.ev_a[src];
-        const uint32_t h = a.hash[i] >> 6;
-        nb += (i == 0 || (a.hash[i - 1] >> 6) != h) ? 1u : 0u;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int i = t0 + s * kNT;
+        src[s] = i < a.n ? a.perm[i] : -1;
+        const uint32_t h = i < a.n ? a.hash[i] >> 6 : 0u;
+        const uint32_t hp = i < a.n && i > 0 ? a.hash[i - 1] >> 6 : ~0u;
+        nb += (i < a.n && (i == 0 || hp != h)) ? 1u : 0u;
     }
+    // four slots' loads in flight together, then their stores
+    auto move4 = [&](int s0) __attribute__((always_inline)) {
+        const int j0 = max(src[s0], 0), j1 = max(src[s0 + 1], 0);
+        const int j2 = max(src[s0 + 2], 0), j3 = max(src[s0 + 3], 0);
+        const float4 p0 = a.pos_a[j0], p1 = a.pos_a[j1], p2 = a.pos_a[j2], p3 = a.pos_a[j3];
+        const float *va = &a.vel_a[0].x;
+        float *vb = &a.vel_b[0].x;
+        const float v0x = va[3 * j0], v0y = va[3 * j0 + 1], v0z = va[3 * j0 + 2];
+        const float v1x = va[3 * j1], v1y = va[3 * j1 + 1], v1z = va[3 * j1 + 2];
+        const float v2x = va[3 * j2], v2y = va[3 * j2 + 1], v2z = va[3 * j2 + 2];
+        const float v3x = va[3 * j3], v3y = va[3 * j3 + 1], v3z = va[3 * j3 + 2];
+        const float4 e0 = a.ev_a[j0], e1 = a.ev_a[j1], e2 = a.ev_a[j2], e3 = a.ev_a[j3];
+        auto put = [&](int i, const float4 &p, float vx, float vy, float vz, const float4 &e)
+            __attribute__((always_inline)) {
+            a.pos_b[i] = p;
+            vb[3 * i] = vx;
+            vb[3 * i + 1] = vy;
+            vb[3 * i + 2] = vz;
+            a.ev_b[i] = e;
+        };
+        const int i = t0 + s0 * kNT;
+        if (src[s0] >= 0) put(i, p0, v0x, v0y, v0z, e0);
+        if (src[s0 + 1] >= 0) put(i + kNT, p1, v1x, v1y, v1z, e1);
+        if (src[s0 + 2] >= 0) put(i + 2 * kNT, p2, v2x, v2y, v2z, e2);
+        if (src[s0 + 3] >= 0) put(i + 3 * kNT, p3, v3x, v3y, v3z, e3);
+    };
+    static_assert(kSlots % 4 == 0, "slots come in fours");
+#pragma unroll
+    for (int s0 = 0; s0 < kSlots; s0 += 4) move4(s0);
     uint32_t tot;
     block_excl_scan<kNT>(nb, s_tmp, tot);
     if (tid == 0) a.bcnt[blockIdx.x] = tot;
@@ -273,14 +321,16 @@ __global__ __launch_bounds__(kNT) void k_cells(ArrangeArgs a) {
         if (lane == 0) s_base[s][wave] = (uint32_t)__popcll(m);
     }
     __syncthreads();
-    if (tid == 0) {
-        uint32_t run = pre;
-        for (int s = 0; s < kSlots; ++s)
-            for (int w = 0; w < kNT / 64; ++w) {
-                const uint32_t c = s_base[s][w];
-                s_base[s][w] = run;
-                run += c;
-            }
+    static_assert(kSlots * (kNT / 64) == 64, "one wave scans the slot counts");
+    if (wave == 0) {   // (slot, wave) order is particle order: one wave scan
+        const uint32_t v = (&s_base[0][0])[lane];
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        (&s_base[0][0])[lane] = pre + x - v;
     }
     __syncthreads();
 #pragma unroll
@@ -449,16 +499,23 @@ __device__ __forceinline__ int cell_req(const ArrangeArgs &a, int c, int n) {
 
 // x-layer starts of an occupied cell: the number of its particles (sorted
 // keys hash[pb, pb + n)) whose micro x layer is < 1, < 2, < 3, packed
-// 16 + 16 | 16 bits.  Small cells are counted in one pass over their keys
-// (independent loads), large ones by binary search.
+// 16 + 16 | 16 bits.  Small cells are counted over their keys, 8 loads in
+// flight at a time; large ones by binary search.
 __device__ __forceinline__ uint2 x_layer_starts(const uint32_t *hash, int pb, int n) {
     int l1 = 0, l2 = 0, l3 = 0;
     if (n <= 32) {
-        for (int j = 0; j < n; ++j) {
-            const uint32_t L = (hash[pb + j] >> 4) & 3u;
-            l1 += L < 1u;
-            l2 += L < 2u;
-            l3 += L < 3u;
+        for (int j0 = 0; j0 < n; j0 += 8) {
+            uint32_t h[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) h[u] = j0 + u < n ? hash[pb + j0 + u] : 0xffffffffu;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t L = (h[u] >> 4) & 3u;
+                const bool in = j0 + u < n;
+                l1 += in && L < 1u;
+                l2 += in && L < 2u;
+                l3 += in && L < 3u;
+            }
         }
     } else {
         l1 = layer_bound(hash, pb, pb + n, 1) - pb;
@@ -482,6 +539,7 @@ __global__ __launch_bounds__(kNT) void k_split_count(ArrangeArgs a) {
         const int n = a.cstart[k + 1] - pb;
         a.xcum[c] = x_layer_starts(a.hash, pb, n);
         const int r = cell_req<kBatch>(a, c, n);
+        a.req[c] = r;
         const int nsms = min(n, r * kSmsTaskParticles);
         st += (uint32_t)(n - nsms);
         ss += (uint32_t)nsms;
@@ -553,15 +611,20 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
     __syncthreads();
     // each thread owns 4 consecutive occupied cells
     const int k0 = blockIdx.x * kSplitTile + tid * (kSplitTile / kNT);
-    int nn[4], rr[4];
+    int nn[4], rr[4], cc[4], pb4[5];
     uint32_t st = 0, ss = 0, sr = 0;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) pb4[s] = k0 + s <= K ? a.cstart[k0 + s] : 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) cc[s] = k0 + s < K ? a.cells[k0 + s] : 0;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) rr[s] = k0 + s < K ? a.req[cc[s]] : 0;   // written by k_split_count
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
         const int k = k0 + s;
         nn[s] = 0;
-        rr[s] = 0;
         if (k < K) {
-            nn[s] = a.cstart[k + 1] - a.cstart[k];
-            rr[s] = cell_req<kBatch>(a, a.cells[k], nn[s]);
+            nn[s] = pb4[s + 1] - pb4[s];
             const int nsms = min(nn[s], rr[s] * kSmsTaskParticles);
             st += (uint32_t)(nn[s] - nsms);
             ss += (uint32_t)nsms;
@@ -575,10 +638,9 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
     for (int s = 0; s < 4; ++s) {
         const int k = k0 + s;
         if (k >= K) break;
-        const int c = a.cells[k];
+        const int c = cc[s];
         const int n = nn[s], r = rr[s];
         const int nsms = min(n, r * kSmsTaskParticles);
-        a.req[c] = r;
         a.tra_base[c] = (int)tra;
         a.sms_base[c] = (int)sms;
         if (r > 0) {
@@ -587,7 +649,7 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
             const uint16_t px = (uint16_t)(local % g.gx);
             const uint16_t py = (uint16_t)(local / g.gx % g.gy);
             const uint16_t pz = (uint16_t)(local / g.gx / g.gy);
-            const int pb = a.cstart[k];
+            const int pb = pb4[s];
             for (int lt = 0; lt < r; ++lt) {
                 const int gi = (int)tb + lt;
                 if (gi >= a.task_cap) {
@@ -651,16 +713,35 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
 // bucket the sorted order.  Block 0 also applies judgeTask's odd-tail rule
 // and writes the reference's d_middle_value_.
 __global__ __launch_bounds__(kNT) void k_compact(ArrangeArgs a) {
-    const int i = blockIdx.x * kNT + threadIdx.x;
+    const int i0 = blockIdx.x * (kNT * kCompactItems) + threadIdx.x;
     const int total_tra = a.dc[kCntTra];
-    if (i < a.n) {
-        const int c = (int)(a.hash[i] >> 6);
-        const int rank = i - a.begin[c];
-        const int r32 = a.req[c] * kSmsTaskParticles;
-        if (rank < r32)
-            a.compact[total_tra + a.sms_base[c] + rank] = i;
-        else
-            a.compact[a.tra_base[c] + rank - r32] = i;
+    // kCompactItems particles per thread, each step's loads issued together
+    int c[kCompactItems], bg[kCompactItems], rq[kCompactItems], sb[kCompactItems],
+        tb[kCompactItems];
+#pragma unroll
+    for (int u = 0; u < kCompactItems; ++u) {
+        const int i = i0 + u * kNT;
+        c[u] = i < a.n ? (int)(a.hash[i] >> 6) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kCompactItems; ++u) {
+        const bool in = i0 + u * kNT < a.n;
+        bg[u] = in ? a.begin[c[u]] : 0;
+        rq[u] = in ? a.req[c[u]] : 0;
+        sb[u] = in ? a.sms_base[c[u]] : 0;
+        tb[u] = in ? a.tra_base[c[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kCompactItems; ++u) {
+        const int i = i0 + u * kNT;
+        if (i < a.n) {
+            const int rank = i - bg[u];
+            const int r32 = rq[u] * kSmsTaskParticles;
+            if (rank < r32)
+                a.compact[total_tra + sb[u] + rank] = i;
+            else
+                a.compact[tb[u] + rank - r32] = i;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const int T = a.dc[kCntT];
@@ -740,7 +821,8 @@ void launch_split_apply(const ArrangeArgs &a, bool batch, hipStream_t s) {
 }
 
 void launch_compact(const ArrangeArgs &a, hipStream_t s) {
-    const int blocks = a.n > 0 ? (a.n + kNT - 1) / kNT : 1;
+    const int per_block = kNT * kCompactItems;
+    const int blocks = a.n > 0 ? (a.n + per_block - 1) / per_block : 1;
     hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kNT), 0, s, a);
 }
 
