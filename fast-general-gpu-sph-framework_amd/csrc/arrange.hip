@@ -3,21 +3,23 @@
 // Replaces Arrangement::arrangeHybridFrame (reference
 // src/grid/sph_arrangement.cu:1493-1506) with a stable, deterministic design:
 //
-//   k_hash        micro hash of every particle + radix-pass-0 tile histograms
-//   k_upsweep     tile histograms of later radix passes
-//   k_rowscan     per digit, exclusive scan of the tile histograms over tiles
-//   k_downsweep   stable scatter (wave64 ballot ranking, tiles in index order)
-//   k_gather      permute pos/vel/ev into the sorted buffer; cell boundaries
+//   k_hash        micro hash of every particle; mover flags (key changed)
+//   k_movers      mover and stayer lists, movers' digit counts
+//   k_msort       one stable radix pass over the movers (decoupled look-back)
+//   k_mplace      final position of every mover
+//   k_merge       sorted order: movers by position, stayers by rank; state
+//                 gathered into the sorted buffer; cell boundaries per tile
 //   k_cells       occupied-cell list (id, first particle), chunk cursors
 //   k_fill        dense cell begin/count (the reference's d_cell_offset_ and
 //                 d_cell_nump_) and per-cell x-layer starts
 //   k_split_*     split heuristic, TRA/SMS counts, task generation + pairing
 //   k_compact     compact index (TRA first), odd-tail pad, middle value
 //
-// The LSD radix sort on (micro id, previous slot) makes the permutation the
-// stable one: ties keep the pre-sort order, which is one of the outcomes the
-// reference's atomicAdd ranks (sph_arrangement.cu:1339) can produce.  Every
-// device-wide scan is reduce-then-scan, so no block waits on another.
+// The sort is stable on (micro id, previous slot): ties keep the pre-sort
+// order, which is one of the outcomes the reference's atomicAdd ranks
+// (sph_arrangement.cu:1339) can produce.  The radix passes over the movers
+// find their tile offsets by look-back with a bounded wait; the other
+// device-wide scans are reduce-then-scan, so there no block waits on another.
 #include "kernels.h"
 
 namespace gsph {
@@ -26,6 +28,8 @@ namespace {
 constexpr int kNT = 256;          // threads of the tile kernels (4 waves)
 constexpr int kItems = 16;        // items per thread: 4096-key tiles
 constexpr int kCompactItems = 4;  // particles per thread of k_compact
+constexpr int kPlaceBlocks = 2048;  // grid of k_mplace (grid-stride over the movers)
+constexpr int kSampleMax = 4096;  // hash[] samples of k_mplace (LDS)
 
 template <bool kBatch>
 __device__ __forceinline__ int scene_of_particle(const SceneGeom *sc, int ns,
@@ -58,135 +62,237 @@ __device__ __forceinline__ int scene_of_cell(const SceneGeom *sc, int ns,
 }
 
 // ---------------------------------------------------------------------------
-// Hash + pass-0 histogram.  Tile t covers keys [4096 t, 4096 t + 4096); wave w
-// of the block owns 1024 consecutive keys, slot s / lane l is key
-// 4096 t + 1024 w + 64 s + l, so (wave, slot, lane) order is index order.
+// The sort, by merging.  Between two steps the state stays in the previous
+// step's sorted order (slot j holds sorted position j), and most particles
+// keep their micro cell, so their new key equals hash[j], the previous sorted
+// key of their slot.  The stable sort of the new keys is the merge, by
+// (key, slot), of two sequences already in that order or made so cheaply:
+// the stayers (key unchanged: in slot order, hence sorted) and the movers
+// (key changed: few, radix sorted on their own).
+//   k_hash     new keys; mover flags (one 64-bit word per 64 slots), the
+//              movers' new keys, movers per tile
+//   k_movers   mover list (key, slot) and stayer list (slot, key), both in
+//              slot order; movers before each flag word; the movers' digit
+//              counts of every radix pass
+//   k_msort    one stable LSD radix pass over the mover list
+//   k_mplace   final position of every mover: its rank among the movers plus
+//              the stayers that precede it (binary search over hash[])
+//   k_merge    every sorted position takes its particle (a mover by position,
+//              a stayer by rank), gathers its state, counts cell boundaries
+// When the slot order is not the previous sorted order (a new context, a
+// state or layout set from the host, an arrange without integrate) the order
+// flag is clear, every particle is a mover and the mover sort is the whole
+// sort.  Either way the result is the stable permutation, bit for bit.
+
+// Tile t of the hash and mover kernels covers slots [4096 t, 4096 t + 4096);
+// wave w owns 1024 consecutive slots, slot s / lane l is 4096 t + 1024 w +
+// 64 s + l, so flag word (4096 t + 1024 w + 64 s) / 64 = 64 t + 16 w + s.
 template <bool kBatch>
 __global__ __launch_bounds__(kNT) void k_hash(ArrangeArgs a) {
-    __shared__ uint32_t s_cnt[4][256];
+    __shared__ uint32_t s_m[kNT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int k = tid; k < 4 * 256; k += kNT) (&s_cnt[0][0])[k] = 0;
-    __syncthreads();
     const int base = blockIdx.x * kSortTile + wave * 1024;
+    const bool sorted = a.dc[kCntOrder] != 0;
+    // all loads first, so that they are in flight together
+    float4 pv[kItems];
+    uint32_t prev[kItems];
+#pragma unroll
+    for (int s = 0; s < kItems; ++s) {
+        const int i = base + s * 64 + lane;
+        pv[s] = i < a.n ? a.pos_a[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        prev[s] = sorted && i < a.n ? a.hash[i] : 0u;
+    }
+    // the wave's 1024 slots usually lie in one scene: look it up once
+    int wave_scene = 0;
+    if (kBatch) {
+        const int last = min(base + 1023, a.n - 1);
+        const int s0 = scene_of_particle<kBatch>(a.scenes, a.nscenes, min(base, last));
+        wave_scene = s0 == scene_of_particle<kBatch>(a.scenes, a.nscenes, last) ? s0 : -1;
+    }
+    const SceneGeom g_wave = a.scenes[wave_scene >= 0 ? wave_scene : 0];
+    uint32_t movers = 0;
+#pragma unroll
     for (int s = 0; s < kItems; ++s) {
         const int i = base + s * 64 + lane;
         const bool valid = i < a.n;
-        uint32_t key = 0;
-        if (valid) {
-            const SceneGeom g = a.scenes[scene_of_particle<kBatch>(a.scenes, a.nscenes, i)];
-            const float4 p = a.pos_a[i];
+        auto key_of = [&](const SceneGeom &g, const float4 &p) __attribute__((always_inline)) {
             int mx, my, mz;
-            if (micro_coords(p.x, p.y, p.z, a.inv_cell_size, g, mx, my, mz)) {
-                key = micro_key(mx, my, mz, g);
-            } else {
-                // Out-of-domain particle: reported to the host, parked in the
-                // scene's first cell so that every later index stays in range.
-                atomicOr(&a.dc[kCntErr], kErrOutOfDomain);
-                key = (uint32_t)g.cell_base << 6;
-            }
-            a.keys[0][i] = key;
+            if (micro_coords(p.x, p.y, p.z, a.inv_cell_size, g, mx, my, mz))
+                return micro_key(mx, my, mz, g);
+            // Out-of-domain particle: reported to the host, parked in the
+            // scene's first cell so that every later index stays in range.
+            atomicOr(&a.dc[kCntErr], kErrOutOfDomain);
+            return (uint32_t)g.cell_base << 6;
+        };
+        bool mv = false;
+        if (valid) {
+            const uint32_t key =
+                wave_scene >= 0 ? key_of(g_wave, pv[s])
+                                : key_of(a.scenes[scene_of_particle<kBatch>(a.scenes, a.nscenes, i)], pv[s]);
+            mv = !sorted || key != prev[s];
+            if (mv) a.mkey[1][i] = key;
         }
-        if (valid) atomicAdd(&s_cnt[wave][key & 255u], 1u);
+        const uint64_t bits = __ballot(mv);
+        if (lane == 0 && base + s * 64 < a.n) a.mflag[(base + s * 64) >> 6] = bits;
+        movers += (uint32_t)__popcll(bits);
     }
+    if (lane == 0) s_m[wave] = movers;
+    if (tid == 0) a.tile_occ[blockIdx.x] = 0;   // k_mplace counts into it
     __syncthreads();
-    const uint32_t tot = s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
-    a.tilecnt[(size_t)tid * a.sort_tiles + blockIdx.x] = tot;
+    if (tid == 0) a.tile_m[blockIdx.x] = s_m[0] + s_m[1] + s_m[2] + s_m[3];
 }
 
-__global__ __launch_bounds__(kNT) void k_upsweep(ArrangeArgs a, int pass) {
-    __shared__ uint32_t s_cnt[4][256];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t *keys = a.keys[pass & 1];
-    const int shift = 8 * pass;
-    for (int k = tid; k < 4 * 256; k += kNT) (&s_cnt[0][0])[k] = 0;
-    const int base = blockIdx.x * kSortTile + wave * 1024;
-    uint32_t kv[kItems];
-#pragma unroll
-    for (int s = 0; s < kItems; ++s) {
-        const int i = base + s * 64 + lane;
-        kv[s] = i < a.n ? keys[i] : 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kItems; ++s) {
-        const int i = base + s * 64 + lane;
-        const bool valid = i < a.n;
-        if (valid) atomicAdd(&s_cnt[wave][(kv[s] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    const uint32_t tot = s_cnt[0][tid] + s_cnt[1][tid] + s_cnt[2][tid] + s_cnt[3][tid];
-    a.tilecnt[(size_t)tid * a.sort_tiles + blockIdx.x] = tot;
-}
-
-// One block per digit: exclusive scan of the digit's row over tiles, in place;
-// the row total goes to digit_total.
-__global__ __launch_bounds__(kNT) void k_rowscan(ArrangeArgs a) {
+// Mover and stayer lists.  A mover's list index is the number of movers in
+// earlier slots; a stayer's (its rank r) the number of stayers in earlier
+// slots.  Also clears the look-back words of this tile and the tickets for
+// the mover sort (no memset node: the step stays capturable as a graph).
+__global__ __launch_bounds__(kNT) void k_movers(ArrangeArgs a) {
     __shared__ uint32_t s_tmp[8];
-    uint32_t *row = a.tilecnt + (size_t)blockIdx.x * a.sort_tiles;
-    uint32_t carry = 0;
-    for (int c0 = 0; c0 < a.sort_tiles; c0 += kNT) {
-        const int t = c0 + threadIdx.x;
-        const uint32_t v = t < a.sort_tiles ? row[t] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan<kNT>(v, s_tmp, tot);
-        if (t < a.sort_tiles) row[t] = carry + ex;
-        carry += tot;
+    __shared__ uint32_t s_wpre[64];          // movers before each flag word of the tile
+    __shared__ uint32_t s_hist[4][256];      // [pass][digit] counts of the tile's movers
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tile = blockIdx.x;
+    const int nwords = (a.n + 63) >> 6;
+    for (int p = 0; p < a.passes; ++p)
+        a.status[((size_t)p * a.sort_tiles + tile) * 256 + tid] = 0ull;
+    if (tile == 0 && tid < 4) a.dc[kCntTicket + tid] = 0;
+    for (int k = tid; k < 4 * 256; k += kNT) (&s_hist[0][0])[k] = 0;
+    // the flag words and keys this block reads, issued early
+    const int base = tile * kSortTile + wave * 1024;
+    uint64_t bits[kItems];
+    uint32_t hv[kItems];
+#pragma unroll
+    for (int s = 0; s < kItems; ++s) {
+        const int i = base + s * 64 + lane;
+        bits[s] = base + s * 64 < a.n ? a.mflag[(base + s * 64) >> 6] : 0ull;
+        hv[s] = i < a.n ? a.hash[i] : 0u;
     }
-    if (threadIdx.x == 0) a.digit_total[blockIdx.x] = carry;
+    uint32_t pre = 0;   // movers before this tile
+    for (int t = tid; t < tile; t += kNT) pre += a.tile_m[t];
+    {
+        uint32_t tot;
+        block_excl_scan<kNT>(pre, s_tmp, tot);
+        pre = tot;
+    }
+    if (wave == 0) {
+        const int w = tile * 64 + lane;
+        const uint32_t c = w < nwords ? (uint32_t)__popcll(a.mflag[w]) : 0u;
+        uint32_t x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        s_wpre[lane] = pre + x - c;
+        if (w < nwords) a.mpre[w] = pre + x - c;
+        if (lane == 63 && tile == (int)gridDim.x - 1) a.dc[kCntMovers] = (int)(pre + x);
+    }
+    __syncthreads();
+    uint32_t mk[kItems];
+#pragma unroll
+    for (int s = 0; s < kItems; ++s) {
+        const int i = base + s * 64 + lane;
+        mk[s] = i < a.n && ((bits[s] >> lane) & 1ull) ? a.mkey[1][i] : 0u;
+    }
+#pragma unroll
+    for (int s = 0; s < kItems; ++s) {
+        const int i = base + s * 64 + lane;
+        if (i < a.n) {
+            const uint32_t before = s_wpre[wave * 16 + s] + (uint32_t)__popcll(bits[s] & lanemask_lt());
+            if ((bits[s] >> lane) & 1ull) {
+                a.mkey[0][before] = mk[s];
+                a.mval[0][before] = i;
+                for (int p = 0; p < a.passes; ++p)
+                    atomicAdd(&s_hist[p][(mk[s] >> (8 * p)) & 255u], 1u);
+            } else {
+                const int r = i - (int)before;
+                a.sidx[r] = i;
+                a.skey[r] = hv[s];
+            }
+        }
+    }
+    __syncthreads();
+    for (int p = 0; p < a.passes; ++p) {
+        const uint32_t v = s_hist[p][tid];
+        if (v) atomicAdd(&a.dhist[p * 256 + tid], v);
+    }
 }
 
-// Stable scatter of one radix pass.  dest = (keys with a smaller digit) +
-// (same digit in earlier tiles) + (same digit earlier in this tile).  The tile
-// is first reordered by digit in LDS, then written out run by run so that
-// consecutive threads store to consecutive addresses.
-template <bool kFirst>
-__global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
+// Look-back word of (pass, tile, digit): state in bits 62-63, count in bits
+// 0-31 (counts stay below n < 2^31).
+constexpr unsigned long long kStCount = 1ull << 62;    // this tile's count
+constexpr unsigned long long kStPrefix = 2ull << 62;   // inclusive prefix over tiles 0..t
+constexpr int kSpinLimit = 1 << 20;                    // stalled rounds before giving up
+constexpr int kLookWindow = 8;                         // predecessor words per round
+
+__device__ __forceinline__ void publish(unsigned long long *w, unsigned long long v) {
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long poll(const unsigned long long *w) {
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One stable radix pass (8-bit digit `pass`) over the mover list, in a single
+// kernel.  Blocks take tickets in launch order and work on tile = ticket, so
+// a tile only waits for tiles that already run; tickets past the list exit.
+// A tile ranks its keys with wave64 lane masks (ties in index order), writes
+// its per-digit counts at once, reorders itself by digit in LDS, and then,
+// one thread per digit, walks back over the earlier tiles' words, adding
+// counts until it meets an inclusive prefix (single-pass scan with decoupled
+// look-back, Merrill & Garland 2016).  dest = (movers with a smaller digit)
+// + (same digit in earlier tiles) + (same digit earlier in this tile).
+// Every word a tile waits on is written by a tile with a smaller ticket
+// before that tile waits on anything, so the walk ends; the number of
+// stalled rounds is bounded all the same, and a tile that reaches the bound
+// raises kErrSortStall and carries on, so the kernel always exits.
+__global__ __launch_bounds__(kNT) void k_msort(ArrangeArgs a, int pass) {
     __shared__ uint32_t s_w[4][256];
     __shared__ uint32_t s_base[256];      // global destination of the tile's digit run
     __shared__ uint32_t s_local[256];     // start of the digit run inside the tile
     __shared__ uint32_t s_tmp[8];
     __shared__ uint32_t s_key[kSortTile];
     __shared__ __align__(8) int s_val[kSortTile];
+    __shared__ int s_tile;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tile = blockIdx.x;
+    const int shift = 8 * pass;
+    const int m = a.dc[kCntMovers];
+    const uint32_t *kin = a.mkey[pass & 1];
+    const int *vin = a.mval[pass & 1];
+    uint32_t *kout = a.mkey[(pass + 1) & 1];
+    int *vout = a.mval[(pass + 1) & 1];
     // lane masks of the digits while the tile is ranked (in s_val's space,
     // which is free until the reorder): 256 words per wave
     unsigned long long *s_peer = reinterpret_cast<unsigned long long *>(s_val);
-    const int shift = 8 * pass;
-    const uint32_t *kin = a.keys[pass & 1];
-    const int *vin = a.vals[pass & 1];
-    uint32_t *kout = a.keys[(pass + 1) & 1];
-    int *vout = a.vals[(pass + 1) & 1];
+    // only as many blocks as the list has tiles take tickets (one atomic
+    // counter: the rest must not queue on it)
+    if ((int)blockIdx.x * kSortTile >= m) return;   // block-uniform
+    if (tid == 0) s_tile = atomicAdd(&a.dc[kCntTicket + pass], 1);
     for (int k = tid; k < 4 * 256; k += kNT) {
         (&s_w[0][0])[k] = 0;
         s_peer[k] = 0ull;
     }
+    uint32_t smaller;   // movers with a smaller digit than tid
     {
         uint32_t tot;
-        const uint32_t ex = block_excl_scan<kNT>(a.digit_total[tid], s_tmp, tot);
-        s_base[tid] = ex + a.tilecnt[(size_t)tid * a.sort_tiles + tile];
+        smaller = block_excl_scan<kNT>(a.dhist[pass * 256 + tid], s_tmp, tot);
     }
-    __syncthreads();
+    const int tile = s_tile;   // ordered by the barriers inside the scan
     const int base = tile * kSortTile + wave * 1024;
-    const int tile_n = min(kSortTile, a.n - tile * kSortTile);
+    const int tile_n = min(kSortTile, m - tile * kSortTile);
     uint32_t key[kItems];
     int val[kItems];
     uint32_t rank[kItems];
 #pragma unroll
     for (int s = 0; s < kItems; ++s) {
         const int i = base + s * 64 + lane;
-        if (i < a.n) {
-            key[s] = kin[i];
-            val[s] = kFirst ? i : vin[i];
-        } else {
-            key[s] = 0xffffffffu;
-            val[s] = 0;
-        }
+        key[s] = i < m ? kin[i] : 0xffffffffu;
+        val[s] = i < m ? vin[i] : 0;
     }
 #pragma unroll
     for (int s = 0; s < kItems; ++s) {
         const int i = base + s * 64 + lane;
-        const bool valid = i < a.n;
+        const bool valid = i < m;
         const uint32_t d = (key[s] >> shift) & 255u;
         // lanes of the wave with the same digit: each sets its bit in the
         // digit's word, then reads the word back (one wave's LDS operations
@@ -217,6 +323,11 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
         }
         cnt_d = run;
     }
+    // publish the count at once, so that later tiles can pass this one
+    // before its own walk is done
+    unsigned long long *st = a.status + (size_t)pass * a.sort_tiles * 256;
+    publish(&st[(size_t)tile * 256 + tid], (tile == 0 ? kStPrefix : kStCount) | cnt_d);
+    // reorder the tile by digit in LDS while the predecessors finish
     {
         uint32_t tot;
         s_local[tid] = block_excl_scan<kNT>(cnt_d, s_tmp, tot);
@@ -225,13 +336,50 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
 #pragma unroll
     for (int s = 0; s < kItems; ++s) {
         const int i = base + s * 64 + lane;
-        if (i < a.n) {
+        if (i < m) {
             const uint32_t d = (key[s] >> shift) & 255u;
             const uint32_t lp = s_local[d] + s_w[wave][d] + rank[s];
             s_key[lp] = key[s];
             s_val[lp] = val[s];
         }
     }
+    uint32_t before = 0;   // movers with digit tid in tiles 0..tile-1
+    if (tile > 0) {
+        // kLookWindow words per round, read together (tiles before 0 read as
+        // an empty prefix); consumed in order up to the first prefix or the
+        // first word not yet written
+        int t = tile - 1, spins = 0;
+        for (;;) {
+            unsigned long long w[kLookWindow];
+#pragma unroll
+            for (int k = 0; k < kLookWindow; ++k)
+                w[k] = t - k >= 0 ? poll(&st[(size_t)(t - k) * 256 + tid]) : kStPrefix;
+            int used = 0;
+            bool done = false;
+#pragma unroll
+            for (int k = 0; k < kLookWindow; ++k) {
+                if (used == k && !done) {
+                    const unsigned long long st_k = w[k] >> 62;
+                    if (st_k != 0) {
+                        before += (uint32_t)w[k];
+                        done = st_k == 2;
+                        used = k + 1;
+                    }
+                }
+            }
+            if (done) break;
+            t -= used;
+            if (used < kLookWindow) {
+                if (++spins > kSpinLimit) {
+                    atomicOr(&a.dc[kCntErr], kErrSortStall);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        publish(&st[(size_t)tile * 256 + tid], kStPrefix | (before + cnt_d));
+    }
+    s_base[tid] = smaller + before;
     __syncthreads();
     for (int k = tid; k < tile_n; k += kNT) {
         const uint32_t kk = s_key[k];
@@ -242,24 +390,193 @@ __global__ __launch_bounds__(kNT) void k_downsweep(ArrangeArgs a, int pass) {
     }
 }
 
-// Permute the state into the sorted buffer and count coarse-cell boundaries
-// (first particle of each occupied cell) per tile.
-__global__ __launch_bounds__(kNT) void k_gather(ArrangeArgs a) {
+// Final position of mover q (sorted list): q + the stayers before it.  The
+// slots s with (hash[s], s) < (key, slot) form a prefix [0, P) of the slot
+// order (hash[] ascends, ties in slot order); the stayers among them are P
+// minus the movers in [0, P).  P is found in a sample of hash[] in LDS (one
+// entry per stride slots), then in the stride-wide range in memory by an
+// 8-way search (7 independent loads per step: a third of the dependent
+// loads of a binary search).  Movers per output tile are counted for k_merge
+// (runs of equal tiles are added once per wave: positions ascend with q).
+__global__ __launch_bounds__(kNT) void k_mplace(ArrangeArgs a) {
+    __shared__ uint32_t s_smp[kSampleMax];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int m = a.dc[kCntMovers];
+    if ((int)blockIdx.x * kNT >= m) return;   // block-uniform
+    const bool sorted = a.dc[kCntOrder] != 0;
+    const uint32_t *mk = a.mkey[a.passes & 1];
+    const int *mv = a.mval[a.passes & 1];
+    int *mdest = reinterpret_cast<int *>(a.mkey[(a.passes & 1) ^ 1]);   // free after the sort
+    const int stride = a.sample_stride;
+    const int ns = (a.n + stride - 1) / stride;
+    if (sorted)
+        for (int k = tid; k < ns; k += kNT) s_smp[k] = a.hash[(size_t)k * stride];
+    __syncthreads();
+    // grid-stride over the list, one mover per thread per round
+    for (int qb = blockIdx.x * kNT; qb < m; qb += gridDim.x * kNT) {
+        const int q = qb + tid;
+        const bool act = q < m;
+        int dest = q;
+        if (act && sorted) {
+            const uint32_t key = mk[q];
+            const int j = mv[q];
+            auto less = [&](uint32_t hk, int s) { return hk < key || (hk == key && s < j); };
+            // samples t < T have (hash[t stride], t stride) < (key, j)
+            int lo = 0, hi = ns;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (less(s_smp[mid], mid * stride))
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            int P;
+            if (lo == 0) {
+                P = 0;
+            } else {
+                // first s in [l, h) with !less(s), or h
+                int l = (lo - 1) * stride + 1, h = min(lo * stride, a.n);
+                while (h - l > 8) {
+                    const int step = (h - l + 7) >> 3;   // 8 parts, the last possibly short
+                    uint32_t hk[7];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) {
+                        const int sk = min(l + (k + 1) * step, h - 1);
+                        hk[k] = a.hash[sk];
+                    }
+                    int nl = l, nh = h;
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) {
+                        const int sk = min(l + (k + 1) * step, h - 1);
+                        if (less(hk[k], sk))
+                            nl = max(nl, sk + 1);
+                        else
+                            nh = min(nh, sk);
+                    }
+                    l = nl;
+                    h = nh;
+                }
+                // at most 8 left: count the prefix that is less, loads together
+                uint32_t hk[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) hk[k] = l + k < h ? a.hash[l + k] : 0u;
+                int c = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) c += (l + k < h && less(hk[k], l + k)) ? 1 : 0;
+                P = l + c;
+            }
+            int movers_before;
+            if (P >= a.n) {
+                movers_before = m;
+            } else {
+                const int w = P >> 6, b = P & 63;
+                const uint64_t below = b == 0 ? 0ull : (~0ull >> (64 - b));
+                movers_before = (int)a.mpre[w] + __popcll(a.mflag[w] & below);
+            }
+            dest = q + (P - movers_before);
+        }
+        if (act) mdest[q] = dest;
+        const int t = act ? dest / kCellTile : -1;
+        const int tp = __shfl_up(t, 1, 64);
+        const bool lead = act && (lane == 0 || tp != t);
+        const uint64_t leads = __ballot(lead);
+        const int nact = __popcll(__ballot(act));   // active lanes are a prefix
+        if (lead) {
+            const uint64_t after = lane == 63 ? 0ull : (leads & (~0ull << (lane + 1)));
+            const int next = after ? __builtin_ctzll(after) : nact;
+            atomicAdd(&a.tile_occ[t], next - lane);
+        }
+    }
+}
+
+// Sorted positions [4096 t, 4096 t + 4096).  The movers placed in the tile
+// are the run q in [q_lo, q_lo + tile_occ[t]) of the sorted list (positions
+// ascend with q); every other position takes the next stayer, rank r = its
+// position minus the movers before it.  Each position gathers its particle's
+// state from the slot order; cell boundaries are counted per tile as the
+// occupied-cell list needs them.  Block 0 also clears the movers' digit
+// counts for the next step and the order flag (hash[] no longer matches the
+// slot order until integrate writes the state back in sorted order).
+__global__ __launch_bounds__(kNT) void k_merge(ArrangeArgs a) {
     __shared__ uint32_t s_tmp[8];
-    const int tid = threadIdx.x;
+    __shared__ uint32_t s_occ[kCellTile / 32];
+    __shared__ uint32_t s_opre[kCellTile / 32];
+    __shared__ uint32_t s_cell[kCellTile];
     constexpr int kSlots = kCellTile / kNT;
-    const int t0 = blockIdx.x * kCellTile + tid;
+    const int tid = threadIdx.x;
+    const int tile = blockIdx.x;
+    const int o0 = tile * kCellTile;
+    if (tile == 0) {
+        for (int k = tid; k < 4 * 256; k += kNT) a.dhist[k] = 0;
+        if (tid == 0) a.dc[kCntOrder] = 0;
+    }
+    const uint32_t *mk = a.mkey[a.passes & 1];
+    const int *mv = a.mval[a.passes & 1];
+    const int *mdest = reinterpret_cast<const int *>(a.mkey[(a.passes & 1) ^ 1]);
+    uint32_t qlo = 0;
+    for (int t = tid; t < tile; t += kNT) qlo += a.tile_occ[t];
+    {
+        uint32_t tot;
+        block_excl_scan<kNT>(qlo, s_tmp, tot);
+        qlo = tot;
+    }
+    const int cnt = a.tile_occ[tile];
+    for (int k = tid; k < kCellTile / 32; k += kNT) s_occ[k] = 0;
+    __syncthreads();
+    for (int k = tid; k < cnt; k += kNT) {
+        const int p = mdest[qlo + k] - o0;
+        atomicOr(&s_occ[p >> 5], 1u << (p & 31));
+    }
+    __syncthreads();
+    {
+        const uint32_t c = tid < kCellTile / 32 ? (uint32_t)__popc(s_occ[tid]) : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan<kNT>(c, s_tmp, tot);
+        if (tid < kCellTile / 32) s_opre[tid] = ex;
+    }
+    __syncthreads();
+    const int r_lo = o0 - (int)qlo;   // stayer rank at the tile's first free position
     int src[kSlots];
-    uint32_t nb = 0;
+    uint32_t key[kSlots];
 #pragma unroll
     for (int s = 0; s < kSlots; ++s) {
-        const int i = t0 + s * kNT;
-        src[s] = i < a.n ? a.perm[i] : -1;
-        const uint32_t h = i < a.n ? a.hash[i] >> 6 : 0u;
-        const uint32_t hp = i < a.n && i > 0 ? a.hash[i - 1] >> 6 : ~0u;
-        nb += (i < a.n && (i == 0 || hp != h)) ? 1u : 0u;
+        const int p = s * kNT + tid;
+        src[s] = -1;
+        key[s] = 0u;
+        if (o0 + p < a.n) {
+            const uint32_t word = s_occ[p >> 5];
+            const int mb = (int)s_opre[p >> 5] + __popc(word & ((1u << (p & 31)) - 1u));
+            if ((word >> (p & 31)) & 1u) {
+                const int q = (int)qlo + mb;
+                src[s] = mv[q];
+                key[s] = mk[q];
+            } else {
+                const int r = r_lo + p - mb;
+                src[s] = a.sidx[r];
+                key[s] = a.skey[r];
+            }
+        }
     }
-    // four slots' loads in flight together, then their stores
+    // the key before the tile's first position, for the boundary test
+    uint32_t kprev = 0u;
+    bool has_prev = false;
+    if (tid == 0 && o0 > 0) {
+        has_prev = true;
+        if (qlo > 0 && mdest[qlo - 1] == o0 - 1)
+            kprev = mk[qlo - 1];
+        else
+            kprev = a.skey[r_lo - 1];
+    }
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int p = s * kNT + tid;
+        if (src[s] >= 0) {
+            s_cell[p] = key[s] >> 6;
+            a.hash[o0 + p] = key[s];
+            a.perm[o0 + p] = src[s];
+        }
+    }
+    // four positions' loads in flight together, then their stores
     auto move4 = [&](int s0) __attribute__((always_inline)) {
         const int j0 = max(src[s0], 0), j1 = max(src[s0 + 1], 0);
         const int j2 = max(src[s0 + 2], 0), j3 = max(src[s0 + 3], 0);
@@ -279,18 +596,29 @@ __global__ __launch_bounds__(kNT) void k_gather(ArrangeArgs a) {
             vb[3 * i + 2] = vz;
             a.ev_b[i] = e;
         };
-        const int i = t0 + s0 * kNT;
+        const int i = o0 + s0 * kNT + tid;
         if (src[s0] >= 0) put(i, p0, v0x, v0y, v0z, e0);
         if (src[s0 + 1] >= 0) put(i + kNT, p1, v1x, v1y, v1z, e1);
         if (src[s0 + 2] >= 0) put(i + 2 * kNT, p2, v2x, v2y, v2z, e2);
         if (src[s0 + 3] >= 0) put(i + 3 * kNT, p3, v3x, v3y, v3z, e3);
     };
-    static_assert(kSlots % 4 == 0, "slots come in fours");
+    static_assert(kSlots % 4 == 0, "positions come in fours");
 #pragma unroll
     for (int s0 = 0; s0 < kSlots; s0 += 4) move4(s0);
+    __syncthreads();
+    uint32_t nb = 0;
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        const int p = s * kNT + tid;
+        if (src[s] >= 0) {
+            const bool first = p == 0 ? (!has_prev || (kprev >> 6) != s_cell[0])
+                                      : s_cell[p - 1] != s_cell[p];
+            nb += first ? 1u : 0u;
+        }
+    }
     uint32_t tot;
     block_excl_scan<kNT>(nb, s_tmp, tot);
-    if (tid == 0) a.bcnt[blockIdx.x] = tot;
+    if (tid == 0) a.bcnt[tile] = tot;
 }
 
 // Occupied-cell list.  The tile prefix is re-derived by every block from the
@@ -779,23 +1107,25 @@ void launch_hash(const ArrangeArgs &a, bool batch, hipStream_t s) {
         hipLaunchKernelGGL(k_hash<false>, dim3(a.sort_tiles), dim3(kNT), 0, s, a);
 }
 
-void launch_upsweep(const ArrangeArgs &a, int pass, hipStream_t s) {
-    hipLaunchKernelGGL(k_upsweep, dim3(a.sort_tiles), dim3(kNT), 0, s, a, pass);
+void launch_movers(const ArrangeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_movers, dim3(a.sort_tiles), dim3(kNT), 0, s, a);
 }
 
-void launch_rowscan(const ArrangeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_rowscan, dim3(256), dim3(kNT), 0, s, a);
+void launch_msort(const ArrangeArgs &a, int pass, hipStream_t s) {
+    hipLaunchKernelGGL(k_msort, dim3(a.sort_tiles), dim3(kNT), 0, s, a, pass);
 }
 
-void launch_downsweep(const ArrangeArgs &a, int pass, hipStream_t s) {
-    if (pass == 0)
-        hipLaunchKernelGGL(k_downsweep<true>, dim3(a.sort_tiles), dim3(kNT), 0, s, a, pass);
-    else
-        hipLaunchKernelGGL(k_downsweep<false>, dim3(a.sort_tiles), dim3(kNT), 0, s, a, pass);
+void launch_mplace(const ArrangeArgs &a, hipStream_t s) {
+    const int blocks = std::min(kPlaceBlocks, (a.n + kNT - 1) / kNT);
+    hipLaunchKernelGGL(k_mplace, dim3(std::max(blocks, 1)), dim3(kNT), 0, s, a);
 }
 
-void launch_gather(const ArrangeArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather, dim3(a.cell_tiles), dim3(kNT), 0, s, a);
+void launch_merge(const ArrangeArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_merge, dim3(a.cell_tiles), dim3(kNT), 0, s, a);
+}
+
+int sample_stride(int n) {
+    return std::max(kSortTile, (n + kSampleMax - 1) / kSampleMax);
 }
 
 void launch_cells(const ArrangeArgs &a, hipStream_t s) {

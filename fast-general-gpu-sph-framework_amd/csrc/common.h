@@ -1,8 +1,9 @@
 // common.h - device-side building blocks shared by the arrange and physics
 // kernels: scene geometry, the bit-exact micro hash, wave64 ballot helpers and
 // block-level scans.  Device-wide scans are reduce-then-scan (per-tile totals
-// written by one kernel, prefixes formed by the next): no block ever waits on
-// another, so no kernel can spin.
+// written by one kernel, prefixes formed by the next), except the radix
+// passes over the moved particles, whose tiles look back at their
+// predecessors' published counts with a bounded wait (k_msort).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,6 +20,7 @@ constexpr float kFloatSmall = 1e-12f;     // reference: src/core/cuda_math.cuh:1
 // Error bits collected in DevCounters::err and reported by the host.
 constexpr int kErrOutOfDomain = 1;
 constexpr int kErrCapacity = 4;
+constexpr int kErrSortStall = 8;     // a look-back wait reached its bound
 
 // Slots of the small device counter block each context keeps.
 enum CounterSlot {
@@ -29,7 +31,10 @@ enum CounterSlot {
     kCntMiddle = 4,     // reference d_middle_value_ (-1 without SMS)
     kCntErr = 5,
     kCntSmsPool = 6,    // rows of the SMS mask pool taken this step
-    kCntSlots = 8,
+    kCntTicket = 8,     // 8..11: tile tickets of the mover radix passes 0..3
+    kCntMovers = 12,    // particles whose key changed this step
+    kCntOrder = 13,     // 1: the slot order is the order of hash[] (set by integrate)
+    kCntSlots = 16,
 };
 
 // Geometry of one scene inside the (possibly batched) virtual grid.  In the

@@ -58,9 +58,13 @@ struct gsph_ctx {
     f3 *vel[2] = {nullptr, nullptr};
     float4 *ev[2] = {nullptr, nullptr};
     f3 *acc = nullptr;
-    uint32_t *keys[2] = {nullptr, nullptr};
-    int *vals[2] = {nullptr, nullptr};
-    uint32_t *tilecnt = nullptr, *digit_total = nullptr, *bcnt = nullptr, *split_sum = nullptr;
+    uint32_t *hash = nullptr, *skey = nullptr;   // sorted keys; stayer keys
+    int *perm = nullptr, *sidx = nullptr;        // sorted -> slot; stayer slots
+    uint32_t *mkey[2] = {nullptr, nullptr};      // mover list (radix ping-pong)
+    int *mval[2] = {nullptr, nullptr};
+    unsigned long long *mflag = nullptr, *status = nullptr;
+    uint32_t *mpre = nullptr, *tile_m = nullptr, *tile_occ = nullptr, *dhist = nullptr;
+    uint32_t *bcnt = nullptr, *split_sum = nullptr;
     int *cells = nullptr, *cstart = nullptr, *chunk_k0 = nullptr;
     int *begin = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
     uint2 *xcum = nullptr;
@@ -206,16 +210,23 @@ ArrangeArgs arrange_args(gsph_ctx *ctx) {
     a.pos_b = ctx->pos[1];
     a.vel_b = ctx->vel[1];
     a.ev_b = ctx->ev[1];
+    a.hash = ctx->hash;
+    a.perm = ctx->perm;
+    a.skey = ctx->skey;
+    a.sidx = ctx->sidx;
     for (int k = 0; k < 2; ++k) {
-        a.keys[k] = ctx->keys[k];
-        a.vals[k] = ctx->vals[k];
+        a.mkey[k] = ctx->mkey[k];
+        a.mval[k] = ctx->mval[k];
     }
+    a.mflag = ctx->mflag;
+    a.mpre = ctx->mpre;
+    a.tile_m = ctx->tile_m;
+    a.tile_occ = ctx->tile_occ;
+    a.dhist = ctx->dhist;
+    a.status = ctx->status;
+    a.sample_stride = sample_stride(ctx->n);
     a.passes = ctx->passes;
     a.sort_tiles = ctx->sort_tiles;
-    a.tilecnt = ctx->tilecnt;
-    a.digit_total = ctx->digit_total;
-    a.hash = ctx->keys[ctx->passes & 1];
-    a.perm = ctx->vals[ctx->passes & 1];
     a.cell_tiles = ctx->cell_tiles;
     a.bcnt = ctx->bcnt;
     a.cells = ctx->cells;
@@ -250,7 +261,7 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.pos_out = ctx->pos[0];
     p.vel_out = ctx->vel[0];
     p.ev_out = ctx->ev[0];
-    p.hash = ctx->keys[ctx->passes & 1];
+    p.hash = ctx->hash;
     p.begin = ctx->begin;
     p.xcum = ctx->xcum;
     p.compact = ctx->compact;
@@ -278,14 +289,13 @@ int do_arrange(gsph_ctx *ctx) {
         { Timed t(ctx, "compact"); launch_compact(a, s); }
     } else {
         { Timed t(ctx, "hash"); launch_hash(a, ctx->batch, s); }
-        { Timed t(ctx, "rowscan"); launch_rowscan(a, s); }
-        { Timed t(ctx, "downsweep"); launch_downsweep(a, 0, s); }
-        for (int p = 1; p < ctx->passes; ++p) {
-            { Timed t(ctx, "upsweep"); launch_upsweep(a, p, s); }
-            { Timed t(ctx, "rowscan"); launch_rowscan(a, s); }
-            { Timed t(ctx, "downsweep"); launch_downsweep(a, p, s); }
+        { Timed t(ctx, "movers"); launch_movers(a, s); }
+        for (int p = 0; p < ctx->passes; ++p) {
+            Timed t(ctx, "msort");
+            launch_msort(a, p, s);
         }
-        { Timed t(ctx, "gather"); launch_gather(a, s); }
+        { Timed t(ctx, "mplace"); launch_mplace(a, s); }
+        { Timed t(ctx, "merge"); launch_merge(a, s); }
         { Timed t(ctx, "cells"); launch_cells(a, s); }
         { Timed t(ctx, "fill"); launch_fill(a, s); }
         { Timed t(ctx, "split_count"); launch_split_count(a, ctx->batch, s); }
@@ -340,6 +350,14 @@ int do_step(gsph_ctx *ctx) {
     return rc;
 }
 
+// The slot order no longer follows hash[] (state or layout set from the
+// host): the next arrange treats every particle as moved.  Host API paths
+// only, never inside a captured step.
+int invalidate_order(gsph_ctx *ctx) {
+    HIP_TRY(ctx, hipMemsetAsync(ctx->dc + kCntOrder, 0, sizeof(int), ctx->stream));
+    return GSPH_OK;
+}
+
 int collect_errors(gsph_ctx *ctx) {
     int flags = 0;
     HIP_TRY(ctx, hipMemcpyAsync(&flags, ctx->dc + kCntErr, sizeof(int),
@@ -356,6 +374,9 @@ int collect_errors(gsph_ctx *ctx) {
         if (flags & kErrCapacity)
             return fail(ctx, GSPH_ERR_CAPACITY,
                         "task capacity exceeded or a cell holds more than 65535 particles");
+        if (flags & kErrSortStall)
+            return fail(ctx, GSPH_ERR_SPIN_TIMEOUT,
+                        "a sort tile waited too long for its predecessor");
     }
     return GSPH_OK;
 }
@@ -434,12 +455,20 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
         ALLOC(ctx->pos[k], N);
         ALLOC(ctx->vel[k], N);
         ALLOC(ctx->ev[k], N);
-        ALLOC(ctx->keys[k], N);
-        ALLOC(ctx->vals[k], N);
+        ALLOC(ctx->mkey[k], N);
+        ALLOC(ctx->mval[k], N);
     }
+    ALLOC(ctx->hash, N);
+    ALLOC(ctx->perm, N);
+    ALLOC(ctx->skey, N);
+    ALLOC(ctx->sidx, N);
+    ALLOC(ctx->mflag, (N + 63) / 64);
+    ALLOC(ctx->mpre, (N + 63) / 64);
+    ALLOC(ctx->tile_m, sort_tiles);
+    ALLOC(ctx->tile_occ, sort_tiles);
+    ALLOC(ctx->dhist, 4 * 256);
+    ALLOC(ctx->status, 4 * 256 * sort_tiles);
     ALLOC(ctx->acc, N);
-    ALLOC(ctx->tilecnt, 256 * sort_tiles);
-    ALLOC(ctx->digit_total, 256);
     ALLOC(ctx->bcnt, cell_tiles);
     ALLOC(ctx->split_sum, 3 * split_tiles);
     ALLOC(ctx->cells, N + 1);
@@ -468,6 +497,7 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->sum_out, 5);
 #undef ALLOC
     HIP_TRY(ctx, hipMemsetAsync(ctx->dc, 0, sizeof(int) * kCntSlots, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->dhist, 0, sizeof(uint32_t) * 4 * 256, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->tasks, 0, sizeof(gsph_task) * task_cap, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return GSPH_OK;
@@ -623,12 +653,13 @@ int capture_graph(gsph_ctx *ctx) {
 namespace gsph {
 
 size_t engine_bytes(long long n, long long cells) {
-    // per particle: 2 x state (44 B), acc, 2 x key/value, cell list, compact,
-    // task slot, pair masks (+ the SMS mask budget); per cell: begin, req,
-    // TRA/SMS bases, x-layer starts
+    // per particle: 2 x state (44 B), acc, sorted keys/perm, stayer and mover
+    // lists, cell list, compact, task slot, pair masks, sort look-back words
+    // (+ the SMS mask budget); per cell: begin, req, TRA/SMS bases, x-layer
+    // starts
     gsph_options o;
     gsph_options_default(&o);
-    return (size_t)(n * (272 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
+    return (size_t)(n * (290 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
 }
 
 int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,
@@ -701,6 +732,7 @@ int engine_recompose(gsph_ctx *src, gsph_ctx *dst, const std::vector<EngineScene
     std::swap(dst->vel[0], dst->vel[1]);
     std::swap(dst->ev[0], dst->ev[1]);
     int rc = configure_layout(dst, geo, (int)n);
+    if (!rc) rc = invalidate_order(dst);
     dst->cur = 0;
     return rc;
 }
@@ -884,6 +916,8 @@ int gsph_set_state(gsph_ctx *ctx, const float *pos4, const float *vel3, const fl
         else
             HIP_TRY(ctx, hipMemsetAsync(ctx->ev[0], 0, sizeof(float4) * n, ctx->stream));
     }
+    int rc = invalidate_order(ctx);
+    if (rc) return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->cur = 0;
     ctx->grid_valid = false;
@@ -983,6 +1017,8 @@ int time_trial(gsph_ctx *ctx, const gsph_options &o, const float4 *pos0, const f
         HIP_TRY(ctx, hipMemcpyAsync(ctx->pos[0], pos0, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
         HIP_TRY(ctx, hipMemcpyAsync(ctx->vel[0], vel0, n * sizeof(f3), hipMemcpyDeviceToDevice, s));
         HIP_TRY(ctx, hipMemcpyAsync(ctx->ev[0], ev0, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+        const int rc0 = invalidate_order(ctx);
+        if (rc0) return rc0;
         ctx->cur = 0;
         ctx->grid_valid = false;
         ctx->masks_valid = false;
@@ -1164,10 +1200,8 @@ int gsph_read_grid(gsph_ctx *ctx, int *hash, int *perm, int *begin, int *count, 
     hipStream_t s = ctx->stream;
     int dc[kCntSlots];
     HIP_TRY(ctx, hipMemcpyAsync(dc, ctx->dc, sizeof(dc), hipMemcpyDeviceToHost, s));
-    if (n && hash)
-        HIP_TRY(ctx, hipMemcpyAsync(hash, ctx->keys[ctx->passes & 1], 4 * n, hipMemcpyDeviceToHost, s));
-    if (n && perm)
-        HIP_TRY(ctx, hipMemcpyAsync(perm, ctx->vals[ctx->passes & 1], 4 * n, hipMemcpyDeviceToHost, s));
+    if (n && hash) HIP_TRY(ctx, hipMemcpyAsync(hash, ctx->hash, 4 * n, hipMemcpyDeviceToHost, s));
+    if (n && perm) HIP_TRY(ctx, hipMemcpyAsync(perm, ctx->perm, 4 * n, hipMemcpyDeviceToHost, s));
     // counts are begin differences (no dense count array on the device)
     std::vector<int> bg_tmp;
     int *bg = nullptr;

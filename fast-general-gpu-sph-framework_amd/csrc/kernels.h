@@ -37,16 +37,23 @@ struct ArrangeArgs {
     float4 *pos_b;
     f3 *vel_b;
     float4 *ev_b;
-    // radix sort
-    uint32_t *keys[2];
-    int *vals[2];
-    int passes;
+    // sort by merging (k_hash .. k_merge)
+    uint32_t *hash;        // [n] sorted keys; kept between steps (previous order)
+    int *perm;             // [n] sorted position -> slot
+    uint32_t *skey;        // [n] stayer list: key, by rank
+    int *sidx;             //     slot, by rank
+    uint32_t *mkey[2];     // [n] mover list keys (radix ping-pong; [1] also
+                           //     holds new keys by slot, later the positions)
+    int *mval[2];          // [n] mover list slots
+    unsigned long long *mflag;   // [ceil(n/64)] mover flags, one bit per slot
+    uint32_t *mpre;        // [ceil(n/64)] movers before each flag word
+    uint32_t *tile_m;      // [sort_tiles] movers per slot tile
+    uint32_t *tile_occ;    // [sort_tiles] movers per sorted tile
+    uint32_t *dhist;       // [4][256] movers' digit counts, zero between steps
+    unsigned long long *status;  // [4][sort_tiles][256] look-back words
+    int sample_stride;     // hash[] sample spacing of k_mplace
+    int passes;            // 8-bit digits of the keys (at most 4)
     int sort_tiles;
-    uint32_t *tilecnt;     // [256][sort_tiles]
-    uint32_t *digit_total; // [256]
-    // results
-    uint32_t *hash;        // sorted keys (== keys[passes & 1])
-    int *perm;             // sorted slot -> previous slot (== vals[passes & 1])
     int cell_tiles;
     uint32_t *bcnt;        // [cell_tiles] boundary counts
     int *cells;            // [n+1] occupied cell ids
@@ -106,10 +113,11 @@ struct PhysicsArgs {
 
 // arrange.hip
 void launch_hash(const ArrangeArgs &a, bool batch, hipStream_t s);
-void launch_upsweep(const ArrangeArgs &a, int pass, hipStream_t s);
-void launch_rowscan(const ArrangeArgs &a, hipStream_t s);
-void launch_downsweep(const ArrangeArgs &a, int pass, hipStream_t s);
-void launch_gather(const ArrangeArgs &a, hipStream_t s);
+void launch_movers(const ArrangeArgs &a, hipStream_t s);
+void launch_msort(const ArrangeArgs &a, int pass, hipStream_t s);
+void launch_mplace(const ArrangeArgs &a, hipStream_t s);
+void launch_merge(const ArrangeArgs &a, hipStream_t s);
+int sample_stride(int n);
 void launch_cells(const ArrangeArgs &a, hipStream_t s);
 void launch_fill(const ArrangeArgs &a, hipStream_t s);
 void launch_split_count(const ArrangeArgs &a, bool batch, hipStream_t s);

@@ -787,6 +787,8 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
 template <bool kBatch>
 __global__ __launch_bounds__(kNT) void k_integrate(PhysicsArgs P) {
     const int i = blockIdx.x * kNT + threadIdx.x;
+    // the state goes back in sorted order: the slot order is hash[]'s again
+    if (i == 0) P.dcw[kCntOrder] = 1;
     if (i >= P.n) return;
     const gsph_constants &c = P.c;
     const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];

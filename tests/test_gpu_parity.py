@@ -181,6 +181,37 @@ def test_ten_step_checksum_and_grid(gsph, po):
     s.close()
 
 
+def test_sort_by_merging_matches_oracle(gsph, po):
+    """After integrate the slot order is the previous sorted order and only
+    the particles whose key changed are sorted again and merged in.  The grid
+    must be the oracle's full stable sort of the state, bit for bit: with few
+    moved particles, with many (fast random velocities), and right after the
+    state is set from the host (every particle treated as moved)."""
+    op, pos = mixed_scene(po)
+    rng = np.random.default_rng(11)
+    for vmax in (0.0, 3.0):
+        banner(f"sort by merging vmax={vmax}")
+        s = make_system(gsph, op, pos)
+        vel = rng.uniform(-vmax, vmax, (len(pos), 3)).astype(np.float32)
+        s.set_state(pos, vel)
+        s.step(3)
+        pg, vg, eg, _ = s.read_state()
+        s.arrange()
+        g = s.read_grid()
+        moved = int(np.count_nonzero(g["perm"] != np.arange(len(pos))))
+        print("slots whose particle changed position", moved, flush=True)
+        compare_grid(g, po.arrange(op, pg))
+        s.density()
+        s.force()
+        s.integrate()
+        # the same state set from the host: a full sort, same grid
+        pg2, vg2, eg2, _ = s.read_state()
+        s.set_state(pg2, vg2, eg2)
+        s.arrange()
+        compare_grid(s.read_grid(), po.arrange(op, pg2))
+        s.close()
+
+
 def test_determinism_and_prune_equivalence(gsph, po):
     op, pos = mixed_scene(po)
     banner("determinism mixed")
@@ -449,4 +480,14 @@ def test_config1_full_size_properties(gsph, po):
     s.sync()
     cs = s.checksum()
     assert np.all(np.isfinite(cs))
+    # later steps sort by merging the moved particles into the previous
+    # order: still the stable sort of the keys of the state in slot order
+    s.step(3)
+    pg, _, _, _ = s.read_state()
+    key, bad = po.hash_(pyoracle.params_from(p), pg)
+    assert bad == 0
+    s.arrange()
+    g = s.read_grid()
+    assert np.array_equal(g["perm"], np.lexsort((np.arange(n), key)))
+    assert np.array_equal(g["hash"], key[g["perm"]])
     s.close()
