@@ -30,6 +30,24 @@ constexpr int kItems = 16;        // items per thread: 4096-key tiles
 constexpr int kCompactItems = 4;  // particles per thread of k_compact
 constexpr int kPlaceBlocks = 2048;  // grid of k_mplace (grid-stride over the movers)
 constexpr int kSampleMax = 4096;  // hash[] samples of k_mplace (LDS)
+constexpr int kMItems = 4;        // movers per thread of k_msort: 1024-mover tiles
+constexpr int kMTile = kNT * kMItems;
+static_assert(kSortTile % kMTile == 0, "mover tiles nest in slot tiles");
+
+// This thread's share of a[0] + ... + a[cnt - 1] (elements tid, tid + kNT,
+// ...), with four loads in flight at a time; the caller reduces over the
+// block.  Used where every block re-derives its prefix from per-tile totals.
+__device__ __forceinline__ uint32_t strided_sum(const uint32_t *a, int cnt) {
+    uint32_t s = 0;
+    for (int t0 = (int)threadIdx.x; t0 < cnt; t0 += 4 * kNT) {
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = t0 + u * kNT < cnt ? a[t0 + u * kNT] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s += v[u];
+    }
+    return s;
+}
 
 template <bool kBatch>
 __device__ __forceinline__ int scene_of_particle(const SceneGeom *sc, int ns,
@@ -149,12 +167,11 @@ __global__ __launch_bounds__(kNT) void k_hash(ArrangeArgs a) {
 __global__ __launch_bounds__(kNT) void k_movers(ArrangeArgs a) {
     __shared__ uint32_t s_tmp[8];
     __shared__ uint32_t s_wpre[64];          // movers before each flag word of the tile
+    __shared__ uint32_t s_wcnt[64];          // movers in each flag word of the tile
     __shared__ uint32_t s_hist[4][256];      // [pass][digit] counts of the tile's movers
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = blockIdx.x;
     const int nwords = (a.n + 63) >> 6;
-    for (int p = 0; p < a.passes; ++p)
-        a.status[((size_t)p * a.sort_tiles + tile) * 256 + tid] = 0ull;
     if (tile == 0 && tid < 4) a.dc[kCntTicket + tid] = 0;
     for (int k = tid; k < 4 * 256; k += kNT) (&s_hist[0][0])[k] = 0;
     // the flag words and keys this block reads, issued early
@@ -167,16 +184,18 @@ __global__ __launch_bounds__(kNT) void k_movers(ArrangeArgs a) {
         bits[s] = base + s * 64 < a.n ? a.mflag[(base + s * 64) >> 6] : 0ull;
         hv[s] = i < a.n ? a.hash[i] : 0u;
     }
-    uint32_t pre = 0;   // movers before this tile
-    for (int t = tid; t < tile; t += kNT) pre += a.tile_m[t];
+    if (lane == 0)
+#pragma unroll
+        for (int s = 0; s < kItems; ++s) s_wcnt[wave * 16 + s] = (uint32_t)__popcll(bits[s]);
+    uint32_t pre = strided_sum(a.tile_m, tile);   // movers before this tile (after the scan)
     {
         uint32_t tot;
         block_excl_scan<kNT>(pre, s_tmp, tot);
         pre = tot;
     }
-    if (wave == 0) {
+    if (wave == 0) {   // s_wcnt is ordered by the barriers of the scan above
         const int w = tile * 64 + lane;
-        const uint32_t c = w < nwords ? (uint32_t)__popcll(a.mflag[w]) : 0u;
+        const uint32_t c = s_wcnt[lane];
         uint32_t x = c;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -218,10 +237,25 @@ __global__ __launch_bounds__(kNT) void k_movers(ArrangeArgs a) {
     }
 }
 
-// Look-back word of (pass, tile, digit): state in bits 62-63, count in bits
-// 0-31 (counts stay below n < 2^31).
+// Look-back word of (pass, tile, digit): state in bits 62-63, the arrange's
+// epoch (31 bits) in bits 31-61, count in bits 0-30 (counts stay below
+// n < 2^31).  A word whose epoch is not the current one was written by an
+// earlier arrange and reads as not yet written, so the words are never
+// cleared (the epoch advances once per arrange; a word would have to stay
+// unwritten for 2^31 arranges to be mistaken for a current one).
 constexpr unsigned long long kStCount = 1ull << 62;    // this tile's count
 constexpr unsigned long long kStPrefix = 2ull << 62;   // inclusive prefix over tiles 0..t
+constexpr unsigned long long kStEpochMask = (1ull << 31) - 1;
+
+__device__ __forceinline__ unsigned long long st_word(unsigned long long state, uint32_t epoch,
+                                                      uint32_t count) {
+    return state | ((epoch & kStEpochMask) << 31) | (count & 0x7fffffffu);
+}
+
+// State of a polled word (0: not written in this arrange).
+__device__ __forceinline__ uint32_t st_state(unsigned long long w, uint32_t epoch) {
+    return ((w >> 31) & kStEpochMask) == (epoch & kStEpochMask) ? (uint32_t)(w >> 62) : 0u;
+}
 constexpr int kSpinLimit = 1 << 20;                    // stalled rounds before giving up
 constexpr int kLookWindow = 8;                         // predecessor words per round
 
@@ -251,22 +285,24 @@ __global__ __launch_bounds__(kNT) void k_msort(ArrangeArgs a, int pass) {
     __shared__ uint32_t s_base[256];      // global destination of the tile's digit run
     __shared__ uint32_t s_local[256];     // start of the digit run inside the tile
     __shared__ uint32_t s_tmp[8];
-    __shared__ uint32_t s_key[kSortTile];
-    __shared__ __align__(8) int s_val[kSortTile];
+    __shared__ __align__(8) uint32_t s_kv[2 * kMTile];   // keys | slots, reordered
+    uint32_t *s_key = s_kv;
+    int *s_val = reinterpret_cast<int *>(s_kv + kMTile);
     __shared__ int s_tile;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int shift = 8 * pass;
     const int m = a.dc[kCntMovers];
+    const uint32_t epoch = (uint32_t)a.dc[kCntEpoch];
     const uint32_t *kin = a.mkey[pass & 1];
     const int *vin = a.mval[pass & 1];
     uint32_t *kout = a.mkey[(pass + 1) & 1];
     int *vout = a.mval[(pass + 1) & 1];
-    // lane masks of the digits while the tile is ranked (in s_val's space,
+    // lane masks of the digits while the tile is ranked (in s_kv's space,
     // which is free until the reorder): 256 words per wave
-    unsigned long long *s_peer = reinterpret_cast<unsigned long long *>(s_val);
+    unsigned long long *s_peer = reinterpret_cast<unsigned long long *>(s_kv);
     // only as many blocks as the list has tiles take tickets (one atomic
     // counter: the rest must not queue on it)
-    if ((int)blockIdx.x * kSortTile >= m) return;   // block-uniform
+    if ((int)blockIdx.x * kMTile >= m) return;   // block-uniform
     if (tid == 0) s_tile = atomicAdd(&a.dc[kCntTicket + pass], 1);
     for (int k = tid; k < 4 * 256; k += kNT) {
         (&s_w[0][0])[k] = 0;
@@ -278,19 +314,19 @@ __global__ __launch_bounds__(kNT) void k_msort(ArrangeArgs a, int pass) {
         smaller = block_excl_scan<kNT>(a.dhist[pass * 256 + tid], s_tmp, tot);
     }
     const int tile = s_tile;   // ordered by the barriers inside the scan
-    const int base = tile * kSortTile + wave * 1024;
-    const int tile_n = min(kSortTile, m - tile * kSortTile);
-    uint32_t key[kItems];
-    int val[kItems];
-    uint32_t rank[kItems];
+    const int base = tile * kMTile + wave * (kMTile / 4);
+    const int tile_n = min(kMTile, m - tile * kMTile);
+    uint32_t key[kMItems];
+    int val[kMItems];
+    uint32_t rank[kMItems];
 #pragma unroll
-    for (int s = 0; s < kItems; ++s) {
+    for (int s = 0; s < kMItems; ++s) {
         const int i = base + s * 64 + lane;
         key[s] = i < m ? kin[i] : 0xffffffffu;
         val[s] = i < m ? vin[i] : 0;
     }
 #pragma unroll
-    for (int s = 0; s < kItems; ++s) {
+    for (int s = 0; s < kMItems; ++s) {
         const int i = base + s * 64 + lane;
         const bool valid = i < m;
         const uint32_t d = (key[s] >> shift) & 255u;
@@ -325,8 +361,8 @@ __global__ __launch_bounds__(kNT) void k_msort(ArrangeArgs a, int pass) {
     }
     // publish the count at once, so that later tiles can pass this one
     // before its own walk is done
-    unsigned long long *st = a.status + (size_t)pass * a.sort_tiles * 256;
-    publish(&st[(size_t)tile * 256 + tid], (tile == 0 ? kStPrefix : kStCount) | cnt_d);
+    unsigned long long *st = a.status + (size_t)pass * a.msort_tiles * 256;
+    publish(&st[(size_t)tile * 256 + tid], st_word(tile == 0 ? kStPrefix : kStCount, epoch, cnt_d));
     // reorder the tile by digit in LDS while the predecessors finish
     {
         uint32_t tot;
@@ -334,7 +370,7 @@ __global__ __launch_bounds__(kNT) void k_msort(ArrangeArgs a, int pass) {
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < kItems; ++s) {
+    for (int s = 0; s < kMItems; ++s) {
         const int i = base + s * 64 + lane;
         if (i < m) {
             const uint32_t d = (key[s] >> shift) & 255u;
@@ -353,15 +389,16 @@ __global__ __launch_bounds__(kNT) void k_msort(ArrangeArgs a, int pass) {
             unsigned long long w[kLookWindow];
 #pragma unroll
             for (int k = 0; k < kLookWindow; ++k)
-                w[k] = t - k >= 0 ? poll(&st[(size_t)(t - k) * 256 + tid]) : kStPrefix;
+                w[k] = t - k >= 0 ? poll(&st[(size_t)(t - k) * 256 + tid])
+                                  : st_word(kStPrefix, epoch, 0u);
             int used = 0;
             bool done = false;
 #pragma unroll
             for (int k = 0; k < kLookWindow; ++k) {
                 if (used == k && !done) {
-                    const unsigned long long st_k = w[k] >> 62;
+                    const uint32_t st_k = st_state(w[k], epoch);
                     if (st_k != 0) {
-                        before += (uint32_t)w[k];
+                        before += (uint32_t)(w[k] & 0x7fffffffu);
                         done = st_k == 2;
                         used = k + 1;
                     }
@@ -377,7 +414,7 @@ __global__ __launch_bounds__(kNT) void k_msort(ArrangeArgs a, int pass) {
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        publish(&st[(size_t)tile * 256 + tid], kStPrefix | (before + cnt_d));
+        publish(&st[(size_t)tile * 256 + tid], st_word(kStPrefix, epoch, before + cnt_d));
     }
     s_base[tid] = smaller + before;
     __syncthreads();
@@ -508,13 +545,15 @@ __global__ __launch_bounds__(kNT) void k_merge(ArrangeArgs a) {
     const int o0 = tile * kCellTile;
     if (tile == 0) {
         for (int k = tid; k < 4 * 256; k += kNT) a.dhist[k] = 0;
-        if (tid == 0) a.dc[kCntOrder] = 0;
+        if (tid == 0) {
+            a.dc[kCntOrder] = 0;
+            a.dc[kCntEpoch] = a.dc[kCntEpoch] + 1;   // the mover passes are done
+        }
     }
     const uint32_t *mk = a.mkey[a.passes & 1];
     const int *mv = a.mval[a.passes & 1];
     const int *mdest = reinterpret_cast<const int *>(a.mkey[(a.passes & 1) ^ 1]);
-    uint32_t qlo = 0;
-    for (int t = tid; t < tile; t += kNT) qlo += a.tile_occ[t];
+    uint32_t qlo = strided_sum(a.tile_occ, tile);
     {
         uint32_t tot;
         block_excl_scan<kNT>(qlo, s_tmp, tot);
@@ -630,8 +669,7 @@ __global__ __launch_bounds__(kNT) void k_cells(ArrangeArgs a) {
     __shared__ uint32_t s_base[kCellTile / kNT][kNT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int kSlots = kCellTile / kNT;
-    uint32_t pre = 0;
-    for (int t = tid; t < (int)blockIdx.x; t += kNT) pre += a.bcnt[t];
+    uint32_t pre = strided_sum(a.bcnt, (int)blockIdx.x);
     {
         uint32_t tot;
         block_excl_scan<kNT>(pre, s_tmp, tot);
@@ -928,10 +966,11 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
     const int K = a.dc[kCntK];
     if (blockIdx.x * kSplitTile >= K) return;
     const int tid = threadIdx.x;
+    uint32_t vs[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) vs[ch] = strided_sum(a.split_sum + ch * a.split_tiles, (int)blockIdx.x);
     for (int ch = 0; ch < 3; ++ch) {
-        uint32_t v = 0;
-        for (int t = tid; t < (int)blockIdx.x; t += kNT)
-            v += a.split_sum[ch * a.split_tiles + t];
+        const uint32_t v = vs[ch];
         uint32_t tot;
         block_excl_scan<kNT>(v, s_tmp, tot);
         if (tid == 0) s_pre[ch] = tot;
@@ -1112,8 +1151,10 @@ void launch_movers(const ArrangeArgs &a, hipStream_t s) {
 }
 
 void launch_msort(const ArrangeArgs &a, int pass, hipStream_t s) {
-    hipLaunchKernelGGL(k_msort, dim3(a.sort_tiles), dim3(kNT), 0, s, a, pass);
+    hipLaunchKernelGGL(k_msort, dim3(a.msort_tiles), dim3(kNT), 0, s, a, pass);
 }
+
+int msort_tiles(int n) { return std::max(1, (n + kMTile - 1) / kMTile); }
 
 void launch_mplace(const ArrangeArgs &a, hipStream_t s) {
     const int blocks = std::min(kPlaceBlocks, (a.n + kNT - 1) / kNT);

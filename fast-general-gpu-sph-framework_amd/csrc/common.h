@@ -34,6 +34,7 @@ enum CounterSlot {
     kCntTicket = 8,     // 8..11: tile tickets of the mover radix passes 0..3
     kCntMovers = 12,    // particles whose key changed this step
     kCntOrder = 13,     // 1: the slot order is the order of hash[] (set by integrate)
+    kCntEpoch = 14,     // arranges so far (tags the sort's look-back words)
     kCntSlots = 16,
 };
 

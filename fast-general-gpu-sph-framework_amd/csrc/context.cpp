@@ -225,6 +225,7 @@ ArrangeArgs arrange_args(gsph_ctx *ctx) {
     a.dhist = ctx->dhist;
     a.status = ctx->status;
     a.sample_stride = sample_stride(ctx->n);
+    a.msort_tiles = msort_tiles(ctx->n);
     a.passes = ctx->passes;
     a.sort_tiles = ctx->sort_tiles;
     a.cell_tiles = ctx->cell_tiles;
@@ -467,7 +468,7 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->tile_m, sort_tiles);
     ALLOC(ctx->tile_occ, sort_tiles);
     ALLOC(ctx->dhist, 4 * 256);
-    ALLOC(ctx->status, 4 * 256 * sort_tiles);
+    ALLOC(ctx->status, 4 * 256 * (size_t)msort_tiles((int)N));
     ALLOC(ctx->acc, N);
     ALLOC(ctx->bcnt, cell_tiles);
     ALLOC(ctx->split_sum, 3 * split_tiles);
@@ -498,6 +499,9 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
 #undef ALLOC
     HIP_TRY(ctx, hipMemsetAsync(ctx->dc, 0, sizeof(int) * kCntSlots, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->dhist, 0, sizeof(uint32_t) * 4 * 256, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->status, 0,
+                                sizeof(unsigned long long) * 4 * 256 * (size_t)msort_tiles((int)N),
+                                ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->tasks, 0, sizeof(gsph_task) * task_cap, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return GSPH_OK;
@@ -659,7 +663,7 @@ size_t engine_bytes(long long n, long long cells) {
     // starts
     gsph_options o;
     gsph_options_default(&o);
-    return (size_t)(n * (290 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
+    return (size_t)(n * (296 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
 }
 
 int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,

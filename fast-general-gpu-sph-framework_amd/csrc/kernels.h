@@ -50,7 +50,8 @@ struct ArrangeArgs {
     uint32_t *tile_m;      // [sort_tiles] movers per slot tile
     uint32_t *tile_occ;    // [sort_tiles] movers per sorted tile
     uint32_t *dhist;       // [4][256] movers' digit counts, zero between steps
-    unsigned long long *status;  // [4][sort_tiles][256] look-back words
+    unsigned long long *status;  // [4][msort_tiles][256] look-back words
+    int msort_tiles;       // tiles of the mover radix passes (1024 movers each)
     int sample_stride;     // hash[] sample spacing of k_mplace
     int passes;            // 8-bit digits of the keys (at most 4)
     int sort_tiles;
@@ -118,6 +119,7 @@ void launch_msort(const ArrangeArgs &a, int pass, hipStream_t s);
 void launch_mplace(const ArrangeArgs &a, hipStream_t s);
 void launch_merge(const ArrangeArgs &a, hipStream_t s);
 int sample_stride(int n);
+int msort_tiles(int n);
 void launch_cells(const ArrangeArgs &a, hipStream_t s);
 void launch_fill(const ArrangeArgs &a, hipStream_t s);
 void launch_split_count(const ArrangeArgs &a, bool batch, hipStream_t s);

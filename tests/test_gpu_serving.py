@@ -57,11 +57,11 @@ def test_results_same_alone_and_in_the_mix(gsph, po):
 def test_budget_defers_and_rejects(gsph, po):
     op = make_params(h=0.02, world=(1, 1, 1))
     reqs = request_scenes(po, k=4, seed=8)
-    # the library's admission estimate: 272 B + the default SMS mask budget (1024 B)
+    # the library's admission estimate: 296 B + the default SMS mask budget (1024 B)
     # per particle, 24 B per cell, 1 MiB
     def est(w, p):
         cells = np.prod([int(np.ceil(np.float32(x) / np.float32(0.02))) for x in w])
-        return len(p) * (272 + 1024) + cells * 24 + (1 << 20)
+        return len(p) * (296 + 1024) + cells * 24 + (1 << 20)
     sizes = [est(w, p) for (w, p, _, _) in reqs]
     s2 = sorted(sizes)
     need = s2[0] + s2[1] - 1                     # never two requests at once ...
