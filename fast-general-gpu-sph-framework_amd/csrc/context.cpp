@@ -321,16 +321,19 @@ int do_density(gsph_ctx *ctx) {
     return rc;
 }
 
-int do_force(gsph_ctx *ctx) {
+// With fuse the force kernel also integrates each particle (the full step);
+// do_integrate(ctx, true) then only does the bookkeeping.
+int do_force(gsph_ctx *ctx, bool fuse = false) {
     if (ctx->n == 0) return GSPH_OK;
     PhysicsArgs p = physics_args(ctx);
     p.use_masks = ctx->masks_valid && ctx->opt.pair_masks != 0;
+    p.fuse_integrate = fuse ? 1 : 0;
     { Timed t(ctx, "force"); launch_force(p, ctx->batch, ctx->stream); }
     return check_launch(ctx, "force");
 }
 
-int do_integrate(gsph_ctx *ctx) {
-    if (ctx->n > 0) {
+int do_integrate(gsph_ctx *ctx, bool fused = false) {
+    if (ctx->n > 0 && !fused) {
         const PhysicsArgs p = physics_args(ctx);
         { Timed t(ctx, "integrate"); launch_integrate(p, ctx->batch, ctx->stream); }
         int rc = check_launch(ctx, "integrate");
@@ -346,8 +349,8 @@ int do_integrate(gsph_ctx *ctx) {
 int do_step(gsph_ctx *ctx) {
     int rc = do_arrange(ctx);
     if (!rc) rc = do_density(ctx);
-    if (!rc) rc = do_force(ctx);
-    if (!rc) rc = do_integrate(ctx);
+    if (!rc) rc = do_force(ctx, true);
+    if (!rc) rc = do_integrate(ctx, true);
     return rc;
 }
 
@@ -977,9 +980,9 @@ int gsph_step(gsph_ctx *ctx, int nsteps, gsph_stage_ms *per_step_ms) {
             (void)hipEventRecord(e[1], ctx->stream);
             if (!rc) rc = do_density(ctx);
             (void)hipEventRecord(e[2], ctx->stream);
-            if (!rc) rc = do_force(ctx);
+            if (!rc) rc = do_force(ctx, true);
             (void)hipEventRecord(e[3], ctx->stream);
-            if (!rc) rc = do_integrate(ctx);
+            if (!rc) rc = do_integrate(ctx, true);
             (void)hipEventRecord(e[4], ctx->stream);
             if (rc) return rc;
             HIP_TRY(ctx, hipEventSynchronize(e[4]));

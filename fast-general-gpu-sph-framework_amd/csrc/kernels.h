@@ -110,6 +110,7 @@ struct PhysicsArgs {
     int sms_pool_rows;
     int *sms_off;          // [task_cap / 2 + 1]
     int *dcw;              // writable alias of dc
+    int fuse_integrate;    // force also integrates (the full step); else k_integrate
 };
 
 // arrange.hip

@@ -561,9 +561,78 @@ __device__ __forceinline__ void force_pair(const float4 &pi, const float4 &ei,
     A.l += Vt * (d2 - 0.75f * t);
 }
 
-__device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, float wi,
-                                            ForceAcc A) {
+// One particle's integration (knIntegrateVelocityE, integration_kernels.cu:
+// 235-336): penalty walls, 1/rho and gravity, acceleration and speed clamps,
+// Euler step, wall clamp, leapfrog evaluated velocity.  Reads the sorted
+// state (p, e, vel vv, acceleration aa) and writes slot i of the output
+// state.  Shared by k_integrate and the force kernel's fused store, so both
+// give the same bits.
+__device__ __forceinline__ void integrate_one(const PhysicsArgs &P, int i, const SceneGeom &g,
+                                              const float4 &p, const float4 &e, const f3 &vv,
+                                              const f3 &aa) {
     const gsph_constants &c = P.c;
+    float v[3] = {vv.x, vv.y, vv.z};
+    float a[3] = {aa.x, aa.y, aa.z};
+    float x[3] = {p.x, p.y, p.z};
+    const float alpha = 20000000.f, beta = 200000.f;
+    const float kern = (float)((double)c.kernel_h + 0.01);
+    const int order[3] = {1, 0, 2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int ax = order[k];
+        float diff = kern - (x[ax] - g.bound_min[ax]);
+        if (diff > 0.f) a[ax] += alpha * diff - beta * v[ax];
+        diff = kern - (g.bound_max[ax] - x[ax]);
+        if (diff > 0.f) a[ax] -= alpha * diff + beta * v[ax];
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) a[ax] = a[ax] * p.w + c.gravity[ax];
+    float sp = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    if (sp > 3000000.f) {
+        const float f = sqrtf(3000000.f / sp);
+        a[0] *= f;
+        a[1] *= f;
+        a[2] *= f;
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) v[ax] += a[ax] * c.time_step;
+    sp = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (sp > 36.f) {
+        const float f = sqrtf(36.f / sp);
+        v[0] *= f;
+        v[1] *= f;
+        v[2] *= f;
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) x[ax] += v[ax] * c.time_step;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        if (x[ax] >= g.bound_max[ax]) {
+            v[ax] *= c.wall_damping;
+            x[ax] = g.bound_max[ax] - kern;
+        }
+        if (x[ax] <= g.bound_min[ax]) {
+            v[ax] *= c.wall_damping;
+            x[ax] = g.bound_min[ax] + kern;
+        }
+    }
+    P.pos_out[i] = make_float4(x[0], x[1], x[2], p.w);
+    f3 vo;
+    vo.x = v[0];
+    vo.y = v[1];
+    vo.z = v[2];
+    P.vel_out[i] = vo;
+    P.ev_out[i] = make_float4((v[0] + e.x) * 0.5f, (v[1] + e.y) * 0.5f,
+                              (v[2] + e.z) * 0.5f, e.w);
+}
+
+// Acceleration of particle i from its sums; with P.fuse_integrate the
+// particle is integrated here too (the full step), else k_integrate does it.
+template <bool kBatch>
+__device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, const float4 &pi,
+                                            const float4 &ei, ForceAcc A) {
+    const gsph_constants &c = P.c;
+    const float wi = pi.w;
     const float fx = A.px * c.half_spiky_value + A.vx * c.viscosity_visco_value;
     const float fy = A.py * c.half_spiky_value + A.vy * c.viscosity_visco_value;
     const float fz = A.pz * c.half_spiky_value + A.vz * c.viscosity_visco_value;
@@ -584,6 +653,10 @@ __device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, float w
     a.y = fy * c.mass + sy;
     a.z = fz * c.mass + sz;
     P.acc[i] = a;
+    if (P.fuse_integrate) {
+        const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+        integrate_one(P, i, g, pi, ei, P.vel[i], a);
+    }
 }
 
 template <bool kBatch, int kStage>
@@ -597,6 +670,8 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     float4 *se = sp + kStage;
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
+    // integrated here: the state goes back in sorted order (see k_integrate)
+    if (P.fuse_integrate && blockIdx.x == 0 && threadIdx.x == 0) P.dcw[kCntOrder] = 1;
     const int tra_tiles = (tra + 63) >> 6;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
@@ -700,7 +775,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 }
                 wave_sync_lds();
             }
-            if (active) force_store(P, i, pi.w, A);
+            if (active) force_store<kBatch>(P, i, pi, ei, A);
         } else {
             const int item = it - tra_tiles;
             const SmsItem sm = sms_item<kBatch>(P, item << 1, T, lane);
@@ -775,7 +850,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                     wave_sync_lds();
                 }
             }
-            if (sm.active) force_store(P, sm.self, pi.w, A);
+            if (sm.active) force_store<kBatch>(P, sm.self, pi, ei, A);
         }
     }
 }
@@ -790,65 +865,8 @@ __global__ __launch_bounds__(kNT) void k_integrate(PhysicsArgs P) {
     // the state goes back in sorted order: the slot order is hash[]'s again
     if (i == 0) P.dcw[kCntOrder] = 1;
     if (i >= P.n) return;
-    const gsph_constants &c = P.c;
     const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
-    const float4 p = P.pos[i];
-    const float4 e = P.ev[i];
-    const f3 vv = P.vel[i];
-    const f3 aa = P.acc[i];
-    float v[3] = {vv.x, vv.y, vv.z};
-    float a[3] = {aa.x, aa.y, aa.z};
-    float x[3] = {p.x, p.y, p.z};
-    const float alpha = 20000000.f, beta = 200000.f;
-    const float kern = (float)((double)c.kernel_h + 0.01);
-    const int order[3] = {1, 0, 2};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int ax = order[k];
-        float diff = kern - (x[ax] - g.bound_min[ax]);
-        if (diff > 0.f) a[ax] += alpha * diff - beta * v[ax];
-        diff = kern - (g.bound_max[ax] - x[ax]);
-        if (diff > 0.f) a[ax] -= alpha * diff + beta * v[ax];
-    }
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) a[ax] = a[ax] * p.w + c.gravity[ax];
-    float sp = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-    if (sp > 3000000.f) {
-        const float f = sqrtf(3000000.f / sp);
-        a[0] *= f;
-        a[1] *= f;
-        a[2] *= f;
-    }
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) v[ax] += a[ax] * c.time_step;
-    sp = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-    if (sp > 36.f) {
-        const float f = sqrtf(36.f / sp);
-        v[0] *= f;
-        v[1] *= f;
-        v[2] *= f;
-    }
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) x[ax] += v[ax] * c.time_step;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        if (x[ax] >= g.bound_max[ax]) {
-            v[ax] *= c.wall_damping;
-            x[ax] = g.bound_max[ax] - kern;
-        }
-        if (x[ax] <= g.bound_min[ax]) {
-            v[ax] *= c.wall_damping;
-            x[ax] = g.bound_min[ax] + kern;
-        }
-    }
-    P.pos_out[i] = make_float4(x[0], x[1], x[2], p.w);
-    f3 vo;
-    vo.x = v[0];
-    vo.y = v[1];
-    vo.z = v[2];
-    P.vel_out[i] = vo;
-    P.ev_out[i] = make_float4((v[0] + e.x) * 0.5f, (v[1] + e.y) * 0.5f,
-                              (v[2] + e.z) * 0.5f, e.w);
+    integrate_one(P, i, g, P.pos[i], P.ev[i], P.vel[i], P.acc[i]);
 }
 
 // knInit (integration_kernels.cu:6-17): ev = 0, acc = 0.
