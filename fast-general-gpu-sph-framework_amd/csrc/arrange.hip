@@ -111,14 +111,17 @@ __global__ __launch_bounds__(kNT) void k_hash(ArrangeArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int base = blockIdx.x * kSortTile + wave * 1024;
     const bool sorted = a.dc[kCntOrder] != 0;
+    // keys computed by the step's fused integration (in mkey[1], by slot)
+    const bool ready = sorted && a.dc[kCntKeysReady] != 0;
     // all loads first, so that they are in flight together
     float4 pv[kItems];
-    uint32_t prev[kItems];
+    uint32_t prev[kItems], nk[kItems];
 #pragma unroll
     for (int s = 0; s < kItems; ++s) {
         const int i = base + s * 64 + lane;
-        pv[s] = i < a.n ? a.pos_a[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        pv[s] = !ready && i < a.n ? a.pos_a[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         prev[s] = sorted && i < a.n ? a.hash[i] : 0u;
+        nk[s] = ready && i < a.n ? a.mkey[1][i] : 0u;
     }
     // the wave's 1024 slots usually lie in one scene: look it up once
     int wave_scene = 0;
@@ -144,11 +147,16 @@ __global__ __launch_bounds__(kNT) void k_hash(ArrangeArgs a) {
         };
         bool mv = false;
         if (valid) {
-            const uint32_t key =
-                wave_scene >= 0 ? key_of(g_wave, pv[s])
-                                : key_of(a.scenes[scene_of_particle<kBatch>(a.scenes, a.nscenes, i)], pv[s]);
-            mv = !sorted || key != prev[s];
-            if (mv) a.mkey[1][i] = key;
+            if (ready) {
+                mv = nk[s] != prev[s];   // the key is in mkey[1] already
+            } else {
+                const uint32_t key =
+                    wave_scene >= 0
+                        ? key_of(g_wave, pv[s])
+                        : key_of(a.scenes[scene_of_particle<kBatch>(a.scenes, a.nscenes, i)], pv[s]);
+                mv = !sorted || key != prev[s];
+                if (mv) a.mkey[1][i] = key;
+            }
         }
         const uint64_t bits = __ballot(mv);
         if (lane == 0 && base + s * 64 < a.n) a.mflag[(base + s * 64) >> 6] = bits;
@@ -547,6 +555,7 @@ __global__ __launch_bounds__(kNT) void k_merge(ArrangeArgs a) {
         for (int k = tid; k < 4 * 256; k += kNT) a.dhist[k] = 0;
         if (tid == 0) {
             a.dc[kCntOrder] = 0;
+            a.dc[kCntKeysReady] = 0;                 // mkey[1] has been reused
             a.dc[kCntEpoch] = a.dc[kCntEpoch] + 1;   // the mover passes are done
         }
     }

@@ -35,6 +35,7 @@ enum CounterSlot {
     kCntMovers = 12,    // particles whose key changed this step
     kCntOrder = 13,     // 1: the slot order is the order of hash[] (set by integrate)
     kCntEpoch = 14,     // arranges so far (tags the sort's look-back words)
+    kCntKeysReady = 15, // 1: next_keys holds the micro ids of the slot order's state
     kCntSlots = 16,
 };
 

@@ -278,6 +278,8 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.sms_pool_rows = ctx->sms_pool_rows;
     p.sms_off = ctx->sms_off;
     p.dcw = ctx->dc;
+    p.fuse_integrate = 0;
+    p.next_keys = ctx->mkey[1];
     return p;
 }
 
@@ -359,6 +361,7 @@ int do_step(gsph_ctx *ctx) {
 // only, never inside a captured step.
 int invalidate_order(gsph_ctx *ctx) {
     HIP_TRY(ctx, hipMemsetAsync(ctx->dc + kCntOrder, 0, sizeof(int), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(ctx->dc + kCntKeysReady, 0, sizeof(int), ctx->stream));
     return GSPH_OK;
 }
 

@@ -111,6 +111,7 @@ struct PhysicsArgs {
     int *sms_off;          // [task_cap / 2 + 1]
     int *dcw;              // writable alias of dc
     int fuse_integrate;    // force also integrates (the full step); else k_integrate
+    uint32_t *next_keys;   // [n] with fuse_integrate: micro id of each integrated particle
 };
 
 // arrange.hip

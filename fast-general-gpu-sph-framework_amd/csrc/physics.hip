@@ -567,9 +567,9 @@ __device__ __forceinline__ void force_pair(const float4 &pi, const float4 &ei,
 // state (p, e, vel vv, acceleration aa) and writes slot i of the output
 // state.  Shared by k_integrate and the force kernel's fused store, so both
 // give the same bits.
-__device__ __forceinline__ void integrate_one(const PhysicsArgs &P, int i, const SceneGeom &g,
-                                              const float4 &p, const float4 &e, const f3 &vv,
-                                              const f3 &aa) {
+__device__ __forceinline__ float3 integrate_one(const PhysicsArgs &P, int i, const SceneGeom &g,
+                                                const float4 &p, const float4 &e, const f3 &vv,
+                                                const f3 &aa) {
     const gsph_constants &c = P.c;
     float v[3] = {vv.x, vv.y, vv.z};
     float a[3] = {aa.x, aa.y, aa.z};
@@ -624,6 +624,7 @@ __device__ __forceinline__ void integrate_one(const PhysicsArgs &P, int i, const
     P.vel_out[i] = vo;
     P.ev_out[i] = make_float4((v[0] + e.x) * 0.5f, (v[1] + e.y) * 0.5f,
                               (v[2] + e.z) * 0.5f, e.w);
+    return make_float3(x[0], x[1], x[2]);
 }
 
 // Acceleration of particle i from its sums; with P.fuse_integrate the
@@ -655,7 +656,17 @@ __device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, const f
     P.acc[i] = a;
     if (P.fuse_integrate) {
         const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
-        integrate_one(P, i, g, pi, ei, P.vel[i], a);
+        const float3 x = integrate_one(P, i, g, pi, ei, P.vel[i], a);
+        // the next arrange's micro id of the new position, as k_hash computes it
+        int mx, my, mz;
+        uint32_t key;
+        if (micro_coords(x.x, x.y, x.z, P.c.inv_cell_size, g, mx, my, mz)) {
+            key = micro_key(mx, my, mz, g);
+        } else {
+            atomicOr(&P.dcw[kCntErr], kErrOutOfDomain);
+            key = (uint32_t)g.cell_base << 6;
+        }
+        P.next_keys[i] = key;
     }
 }
 
@@ -671,7 +682,11 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
     // integrated here: the state goes back in sorted order (see k_integrate)
-    if (P.fuse_integrate && blockIdx.x == 0 && threadIdx.x == 0) P.dcw[kCntOrder] = 1;
+    // and next_keys holds its micro ids
+    if (P.fuse_integrate && blockIdx.x == 0 && threadIdx.x == 0) {
+        P.dcw[kCntOrder] = 1;
+        P.dcw[kCntKeysReady] = 1;
+    }
     const int tra_tiles = (tra + 63) >> 6;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
@@ -863,10 +878,14 @@ template <bool kBatch>
 __global__ __launch_bounds__(kNT) void k_integrate(PhysicsArgs P) {
     const int i = blockIdx.x * kNT + threadIdx.x;
     // the state goes back in sorted order: the slot order is hash[]'s again
-    if (i == 0) P.dcw[kCntOrder] = 1;
+    // (the next keys are not computed here)
+    if (i == 0) {
+        P.dcw[kCntOrder] = 1;
+        P.dcw[kCntKeysReady] = 0;
+    }
     if (i >= P.n) return;
     const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
-    integrate_one(P, i, g, P.pos[i], P.ev[i], P.vel[i], P.acc[i]);
+    (void)integrate_one(P, i, g, P.pos[i], P.ev[i], P.vel[i], P.acc[i]);
 }
 
 // knInit (integration_kernels.cu:6-17): ev = 0, acc = 0.
