@@ -1111,7 +1111,10 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
         gsph_options o = shape;
         o.split_mode = 0;
         trial(o);
-        const int rules[][2] = {{15, 96}, {24, 128}, {32, 192}, {48, 256}, {64, 384}};
+        // tuned (n, 7-cell total) thresholds, the last one "never SMS" (every
+        // cell on the TRA path, with pool masks for long rows)
+        const int rules[][2] = {{15, 96}, {24, 128}, {32, 192}, {48, 256}, {64, 384},
+                                {1 << 30, 1 << 30}};
         for (const auto &r : rules) {
             o = shape;
             o.split_mode = 1;

@@ -155,6 +155,27 @@ __device__ __forceinline__ int wave_min(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
+__device__ __forceinline__ int wave_max_neg(int v) {   // max of v >= -2^31 + 1, kNone-free
+    return -wave_min(-v);
+}
+
+// Rows of the mask pool for one wave (lane 0 takes them with one atomic;
+// -1 when the pool cannot hold them).  No increment once the pool is full, so
+// the counter cannot run past it by more than one row set per wave.
+__device__ __forceinline__ int pool_take(const PhysicsArgs &P, int rows, int cap, int lane) {
+    int b = -1;
+    if (lane == 0) {
+        const int limit = cap - rows;
+        if (rows > 0 && limit >= 0 &&
+            __hip_atomic_load(&P.dcw[kCntSmsPool], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <=
+                limit) {
+            b = atomicAdd(&P.dcw[kCntSmsPool], rows);
+            if (b < 0 || b > limit) b = -1;
+        }
+    }
+    return __builtin_amdgcn_readfirstlane(b);
+}
+
 // candidates staged per wave and round: a launch option (64, 128 or 256)
 constexpr int kStageDefault = 128;
 constexpr int kNone = 0x7fffffff;
@@ -193,7 +214,12 @@ __device__ __forceinline__ float dist2(const float4 &a, const float4 &b) {
 // i the particle's sorted index (stored only when every row fits; an empty
 // row is stored as 0).  kMaskWords words per entry in all.
 constexpr int kMaskRowMax = 32;
-constexpr uint32_t kFlagLongRow = 1u, kFlagPlan = 2u;
+// bit 2 (kFlagPool): the tile has a row longer than one word and keeps its
+// masks in the mask pool instead, one word per 32 positions of each staging
+// round (rounds in walk order; rows of 64 words, one per lane); the pool row
+// of the tile's first word is the flag word >> 3
+constexpr uint32_t kFlagLongRow = 1u, kFlagPlan = 2u, kFlagPool = 4u;
+constexpr int kPoolBaseMax = 1 << 29;
 
 __device__ __forceinline__ uint32_t finish_mask(uint32_t shifted, int cnt) {
     // bits entered at the bottom in candidate order; reverse into bit k
@@ -284,30 +310,30 @@ __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi,
     }
 }
 
-// Wave-cooperative walk over one row: the lanes' ranges [lo, hi) are covered
-// by rounds of kStage consecutive particles loaded into LDS with coalesced
-// loads (gaps between lanes' ranges are skipped); each lane then reads its
-// own candidates from LDS in increasing index order.  visit(jb, je, base)
-// is called on every lane with [jb, je) = [lo, hi) clipped to
-// [base, base + kStage) (empty for lanes without candidates this round).
-template <int kStage, class Stage, class Visit>
-__device__ __forceinline__ void staged_row(int lo, int hi, int lane, Stage &&stage,
-                                           Visit &&visit) {
-    int cur = wave_min(lo < hi ? lo : kNone);
-    while (cur != kNone) {
-        stage(cur);
-        wave_sync_lds();
-        const int jb = max(lo, cur), je = min(hi, cur + kStage);
-        visit(jb, je, cur);
-        wave_sync_lds();
-        const int nxt = hi > cur + kStage ? max(lo, cur + kStage) : kNone;
-        cur = wave_min(nxt);
+// Upper bound of a TRA tile's staging rounds (next_round below): each live
+// row's rounds start at the wave's lowest lo and advance by at least kStage
+// while some lane's range reaches past the round, so a row takes at most
+// ceil((max hi - min lo) / kStage).  Wave-uniform; all lanes must call it.
+template <int kStage>
+__device__ __forceinline__ int tra_round_bound(const int lo[9], const int hi[9]) {
+    int rounds = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const bool live = lo[r] < hi[r];
+        const int lmin = wave_min(live ? lo[r] : kNone);
+        if (lmin != kNone) {
+            const int hmax = wave_max_neg(live ? hi[r] : lmin);
+            rounds += (hmax - lmin + kStage - 1) / kStage;
+        }
     }
-    (void)lane;
+    return rounds;
 }
 
 // All 9 rows of a TRA tile walked as one sequence of staging rounds (rows in
-// order; within a row as in staged_row), software-pipelined by the callers:
+// order; within a row, the lanes' ranges [lo, hi) are covered by rounds of
+// kStage consecutive particles loaded into LDS with coalesced loads, gaps
+// between lanes' ranges skipped; each lane reads its own candidates from LDS
+// in index order), software-pipelined by the callers:
 // while round t is visited, the loads of round t + 1 (possibly the next
 // row's first) are in flight.  Rows empty for every lane are skipped.  The
 // row bounds come from LDS (rlo/rhi: row r of lane l at r * 64 + l), so the
@@ -318,7 +344,11 @@ template <int kStage>
 __device__ __forceinline__ void next_round(int &r, int &cur, int &lo_r, int &hi_r,
                                            const int *rlo, const int *rhi, int lane) {
     int nxt = kNone;
-    if (r >= 0) nxt = wave_min(hi_r > cur + kStage ? max(lo_r, cur + kStage) : kNone);
+    // only lanes with candidates in the row steer the walk, so the rounds
+    // depend on the live ranges alone (density and force must walk the same
+    // rounds: the pool masks are indexed by round)
+    if (r >= 0)
+        nxt = wave_min(lo_r < hi_r && hi_r > cur + kStage ? max(lo_r, cur + kStage) : kNone);
     while (nxt == kNone && r < 8) {
         ++r;
         lo_r = rlo[r * 64 + lane];
@@ -383,6 +413,22 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 rlo[r * 64 + lane] = lo[r];
                 rhi[r * 64 + lane] = hi[r];
             }
+            // a tile with a row longer than one mask word keeps its masks in
+            // the pool: rows for an upper bound of its staging rounds (each
+            // row's rounds advance by at least kStage over the wave's span)
+            int pbase = -1, pool_rounds = 0;
+            if (P.use_masks) {
+                bool lng = false;
+#pragma unroll
+                for (int r = 0; r < 9; ++r) lng |= active && hi[r] - lo[r] > kMaskRowMax;
+                if (__ballot(lng) != 0) {
+                    pool_rounds = tra_round_bound<kStage>(lo, hi);
+                    pbase = pool_take(P, pool_rounds * (kStage / 32), min(P.sms_pool_rows, kPoolBaseMax),
+                                      lane);
+                }
+            }
+            const bool pool = pbase >= 0;
+            int round = 0;
             // per-row partial sums (rejected candidates add exact zeros, so
             // pruning cannot change the result) and per-row accept masks
             float s = 0.f, rs = 0.f;
@@ -393,7 +439,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 for (; done < upto; ++done) {
                     s += rs;
                     rs = 0.f;
-                    if (P.use_masks) {
+                    if (P.use_masks && !pool) {
                         const int cnt = rhi[done * 64 + lane] - rlo[done * 64 + lane];
                         long_row |= cnt > kMaskRowMax;
                         if (active)
@@ -416,7 +462,29 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 finish_rows(rr);
                 const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
                 const int cnt = je - jb;
-                if (cnt > 0) {
+                if (pool) {
+                    // the same candidates in the same order, 32 window
+                    // positions per mask word
+#pragma unroll
+                    for (int c = 0; c < kStage / 32; ++c) {
+                        const int cb = max(jb, base + 32 * c), ce = min(je, base + 32 * c + 32);
+                        const int nc = ce - cb;
+                        uint32_t w = 0;
+                        if (nc > 0) {
+                            const float4 *q = stg + (cb - base);
+                            int k = 0;
+                            for (; k + 4 <= nc; k += 4) {
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, w);
+                            }
+                            for (; k < nc; ++k) density_pair(pi, q[k], h2, pr, rs, w);
+                            w = finish_mask(w, nc) << (cb - base - 32 * c);
+                        }
+                        if (round < pool_rounds)   // always, by the bound
+                            P.sms_pool[(size_t)(pbase + round * (kStage / 32) + c) * 64 + lane] = w;
+                    }
+                    ++round;
+                } else if (cnt > 0) {
                     const float4 *q = stg + (jb - base);
                     int k = 0;
                     for (; k + 4 <= cnt; k += 4) {
@@ -440,7 +508,8 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                             ((uint32_t)off & 0xffffffu) | ((uint32_t)len << 24);
                     }
                     P.nbmask[(size_t)9 * P.mask_stride + idx] =
-                        (long_row ? kFlagLongRow : 0u) | (fits ? kFlagPlan : 0u);
+                        pool ? (kFlagPool | ((uint32_t)pbase << 3) | (fits ? kFlagPlan : 0u))
+                             : ((long_row ? kFlagLongRow : 0u) | (fits ? kFlagPlan : 0u));
                 }
             }
         } else {
@@ -714,15 +783,18 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 tra_rows<kBatch>(P, pi, active, i, lo, hi);
             }
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            // lanes whose density pass recorded masks walk their accepted
-            // pairs in one flat loop (rows in order, candidates in order);
-            // the others test their rows here, in the same order
             // each row is staged through LDS; lanes whose density pass
             // recorded the row's accepted-pair mask run the pair body over
             // it, the others (masks off, or a row longer than one mask word)
             // test the staged candidates first.  Same pairs, same order.
+            // Tiles with long rows read their masks per staging round from
+            // the pool (lane 0 is active in every tile: its flag is the tile's).
+            const uint32_t flag0 = __builtin_amdgcn_readfirstlane(flag);
+            const bool pool = P.use_masks && (flag0 & kFlagPool) != 0;
+            const int pbase = (int)(flag0 >> 3);
+            const int pool_rounds = pool ? tra_round_bound<kStage>(lo, hi) : 0;   // density's bound
             const bool self_test = !P.use_masks || (active && (flag & kFlagLongRow) != 0);
-            const bool use_mask = P.use_masks && active && !self_test;
+            const bool use_mask = P.use_masks && active && !self_test && !pool;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
                 rlo[r * 64 + lane] = lo[r];
@@ -732,12 +804,22 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             uint32_t m_ld = 0, m_st = 0;   // row mask of the loaded / the staged round
             int r_ld = -1;
             int r = -1, cur = kNone, lo_r = 0, hi_r = 0;
+            uint32_t pw_ld[kStage / 32], pw_st[kStage / 32];   // pool words of a round
+            int round_ld = 0;
             auto load = [&]() __attribute__((always_inline)) {
                 load_round<kStage>(P.pos, cur, P.n, lane, p0, p1, p2, p3);
                 load_round<kStage>(P.ev, cur, P.n, lane, e0, e1, e2, e3);
                 if (r != r_ld) {
                     r_ld = r;
                     m_ld = use_mask ? P.nbmask[(size_t)r * P.mask_stride + idx] : 0u;
+                }
+                if (pool) {
+#pragma unroll
+                    for (int c = 0; c < kStage / 32; ++c)
+                        pw_ld[c] = round_ld < pool_rounds
+                                       ? P.sms_pool[(size_t)(pbase + round_ld * (kStage / 32) + c) * 64 + lane]
+                                       : 0u;
+                    ++round_ld;
                 }
             };
             next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
@@ -747,13 +829,27 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 store_round<kStage>(sp, lane, p0, p1, p2, p3);
                 store_round<kStage>(se, lane, e0, e1, e2, e3);
                 m_st = m_ld;
+#pragma unroll
+                for (int c = 0; c < kStage / 32; ++c) pw_st[c] = pool ? pw_ld[c] : 0u;
                 wave_sync_lds();
                 next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
                 if (cur != kNone) load();
                 const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
                 const float4 *q = sp + (jb - base);
                 const float4 *qe = se + (jb - base);
-                if (!self_test) {
+                if (pool) {
+                    // bit k of word c: window position 32 c + k (zero outside
+                    // this lane's range)
+#pragma unroll
+                    for (int c = 0; c < kStage / 32; ++c) {
+                        uint32_t mm = pw_st[c];
+                        while (mm) {
+                            const int kk = 32 * c + __builtin_ctz(mm);
+                            mm &= mm - 1u;
+                            force_pair(pi, ei, sp[kk], se[kk], P.c, h2d, A);
+                        }
+                    }
+                } else if (!self_test) {
                     const int cnt = je - jb;   // <= 32, and jb - lo < 32
                     uint32_t mm = 0;
                     if (cnt > 0) {
