@@ -1085,7 +1085,8 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
         hipMemcpyAsync(ev0, ctx->ev[0], n * sizeof(float4), hipMemcpyDeviceToDevice, s) != hipSuccess)
         rc = fail(ctx, GSPH_ERR_HIP, "autotune snapshot copy failed");
     // coordinate search: launch shape (grid x staging depth x pair masks)
-    // under the current split, then the split rule under the best shape
+    // under the current split, then the split rule under the best shape, then
+    // the shape again if the split changed
     gsph_options best = start;
     float best_t = 1e30f;
     auto trial = [&](const gsph_options &o) {
@@ -1097,15 +1098,18 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
             best = o;
         }
     };
-    for (int bpc : {4, 8, 16, 32})
-        for (int depth : {64, 128, 256})
-            for (int masks : {1, 0}) {
-                gsph_options o = start;
-                o.blocks_per_cu = bpc;
-                o.stage_depth = depth;
-                o.pair_masks = masks;
-                trial(o);
-            }
+    auto shape_search = [&](const gsph_options split) {   // a copy: trial() updates best
+        for (int bpc : {4, 8, 16, 32})
+            for (int depth : {64, 128, 256})
+                for (int masks : {1, 0}) {
+                    gsph_options o = split;
+                    o.blocks_per_cu = bpc;
+                    o.stage_depth = depth;
+                    o.pair_masks = masks;
+                    trial(o);
+                }
+    };
+    shape_search(start);
     if (tune_split) {
         const gsph_options shape = best;
         gsph_options o = shape;
@@ -1122,6 +1126,10 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
             o.sms_min_total = r[1];
             trial(o);
         }
+        // a different split moves work between the paths: tune the shape again
+        if (best.split_mode != shape.split_mode || best.sms_min_self != shape.sms_min_self ||
+            best.sms_min_total != shape.sms_min_total)
+            shape_search(best);
     }
     (void)hipFree(pos0);
     (void)hipFree(vel0);
