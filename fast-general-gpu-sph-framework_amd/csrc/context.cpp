@@ -1268,6 +1268,16 @@ int gsph_checksum(gsph_ctx *ctx, double out[5]) {
     return gsph_checksum_range(ctx, 0, ctx->n, out);
 }
 
+int gsph_mask_pool_info(gsph_ctx *ctx, int *used_rows, int *capacity_rows) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    int dc[kCntSlots];
+    HIP_TRY(ctx, hipMemcpyAsync(dc, ctx->dc, sizeof(dc), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (used_rows) *used_rows = std::min(dc[kCntSmsPool], ctx->sms_pool_rows);
+    if (capacity_rows) *capacity_rows = ctx->sms_pool_rows;
+    return GSPH_OK;
+}
+
 int gsph_get_pair_stats(gsph_ctx *ctx, gsph_pair_stats *out) {
     if (!ctx || !out) return fail(ctx, GSPH_ERR_INVALID_ARG, "null argument");
     std::memset(out, 0, sizeof(*out));

@@ -149,6 +149,7 @@ _SIGS = {
     "gsph_sync": (_I, [_P]),
     "gsph_read_state": (_I, [_P, _P, _P, _P, _P]),
     "gsph_grid_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "gsph_mask_pool_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "gsph_read_grid": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "gsph_checksum": (_I, [_P, _P]),
     "gsph_checksum_range": (_I, [_P, _I, _I, _P]),
@@ -384,6 +385,12 @@ class System:
         self._c(self._lib.gsph_grid_info(self.h, C.byref(cells), C.byref(tasks), C.byref(mid),
                                          C.byref(tra)))
         return dict(cells=cells.value, tasks=tasks.value, middle_raw=mid.value, tra=tra.value)
+
+    def mask_pool_info(self):
+        """(rows taken by the last density pass, capacity) of the mask pool."""
+        used, cap = C.c_int(), C.c_int()
+        self._c(self._lib.gsph_mask_pool_info(self.h, C.byref(used), C.byref(cap)))
+        return used.value, cap.value
 
     def read_grid(self):
         info = self.grid_info()

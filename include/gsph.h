@@ -297,6 +297,11 @@ int gsph_checksum(gsph_ctx *ctx, double out[5]);
 int gsph_checksum_range(gsph_ctx *ctx, int first, int count, double out[5]);
 /* Arrangement::getSmsTaskPairStats (sph_arrangement.cu:1595-1636). */
 int gsph_get_pair_stats(gsph_ctx *ctx, gsph_pair_stats *out);
+/* Accepted-pair mask pool: rows (64 words each) taken by the last density
+ * pass (SMS items and TRA tiles with rows longer than one mask word) and the
+ * capacity (sms_mask_bytes per particle).  No counterpart in the reference,
+ * which has no accepted-pair masks; a diagnostic for tuning sms_mask_bytes. */
+int gsph_mask_pool_info(gsph_ctx *ctx, int *used_rows, int *capacity_rows);
 /* Per-kernel device time (ms, summed) accumulated while kernel timing is on;
  * names are written as a ';'-separated list. */
 int gsph_kernel_timing(gsph_ctx *ctx, int enable);

@@ -233,10 +233,10 @@ ALL_TRA = {"split_mode": 1, "sms_min_self": 1 << 20, "sms_min_total": 1 << 20}
                                        ("odd_tasks", {}), ("dense", {"sms_mask_bytes": 16})])
 def test_pair_masks_change_nothing(gsph, po, name, opts):
     """Force walking density's accepted-pair masks gives the same bits as force
-    testing every candidate itself (dense all-TRA rows exceed one mask word,
-    so those particles take the self-testing path inside the same waves).
-    SMS items record their masks in a pool; a pool too small for all items
-    (16 B per particle) makes the rest test their candidates in force."""
+    testing every candidate itself.  Dense all-TRA rows exceed one mask word:
+    those tiles record per-round masks in the mask pool, as SMS items do; a
+    pool too small for all of them (16 B per particle) makes the rest test
+    their candidates in force."""
     op, pos = scene(po, name)
     banner(f"pair masks {name} {opts}")
     out = []
@@ -244,6 +244,11 @@ def test_pair_masks_change_nothing(gsph, po, name, opts):
         s = make_system(gsph, op, pos, pair_masks=masks, **opts)
         s.step(3)
         out.append(s.read_state())
+        used, cap = s.mask_pool_info()
+        if masks and opts == ALL_TRA:
+            # dense all-TRA rows exceed one mask word: those tiles keep their
+            # masks per staging round in the pool, so the pool path ran
+            assert used > 0, (used, cap)
         s.close()
     for k in range(4):
         assert np.array_equal(out[0][k], out[1][k])
