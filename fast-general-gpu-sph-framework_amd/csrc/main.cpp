@@ -2,24 +2,58 @@
 //   gsph --benchmark N | --headless N   run N frames and print the report
 //   --scene PATH                        reference scene JSON (default assets/scene_default.json)
 //   --config K                          synthetic workload K (0, 1, 2) instead of a JSON scene
+//   --autotune                          pick launch options and the SMS/TRA split by timing
+//                                       the scene first (no counterpart in the reference)
+#include <unistd.h>
+
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <string>
 
 #include "gsph.h"
 #include "hybrid_system.h"
 
+namespace {
+
+// The reference reads assets/ relative to the working directory.  When that
+// file is absent (and no --scene was given), the copy shipped beside the
+// executable's package (<exe dir>/../assets/...) is used instead.
+std::string default_scene(const std::string &path) {
+    if (std::ifstream(path).good()) return path;
+    char buf[4096];
+    const ssize_t k = readlink("/proc/self/exe", buf, sizeof(buf) - 1);
+    if (k <= 0) return path;
+    buf[k] = '\0';
+    const std::string exe(buf);
+    const size_t slash = exe.rfind('/');
+    if (slash == std::string::npos) return path;
+    const std::string alt = exe.substr(0, slash) + "/../" + path;
+    return std::ifstream(alt).good() ? alt : path;
+}
+
+}  // namespace
+
 int main(int argc, char **argv) {
     int frames = 200;  // headless default (main.cpp:44)
     sph::SceneSource src;
-    for (int i = 1; i + 1 < argc; ++i) {
-        if (!std::strcmp(argv[i], "--headless") || !std::strcmp(argv[i], "--benchmark"))
+    bool scene_given = false, autotune = false;
+    for (int i = 1; i < argc; ++i) {
+        const bool has_value = i + 1 < argc;
+        if (!std::strcmp(argv[i], "--autotune")) {
+            autotune = true;
+        } else if (has_value &&
+                   (!std::strcmp(argv[i], "--headless") || !std::strcmp(argv[i], "--benchmark"))) {
             frames = std::atoi(argv[++i]);
-        else if (!std::strcmp(argv[i], "--scene"))
+        } else if (has_value && !std::strcmp(argv[i], "--scene")) {
             src.json_path = argv[++i];
-        else if (!std::strcmp(argv[i], "--config"))
+            scene_given = true;
+        } else if (has_value && !std::strcmp(argv[i], "--config")) {
             src.config = std::atoi(argv[++i]);
+        }
     }
+    if (!scene_given) src.json_path = default_scene(src.json_path);
     if (frames <= 0) {
         std::fprintf(stderr, "Benchmark frame count must be positive.\n");
         return EXIT_FAILURE;
@@ -34,6 +68,18 @@ int main(int argc, char **argv) {
     const sph::float3h side{80.f, 80.f, 80.f};
     sph::HybridSystem system(side, origin, true, src);
     if (!system.ok()) return EXIT_FAILURE;
+    if (autotune) {
+        gsph_options chosen;
+        float ms = 0.f;
+        if (gsph_autotune(system.context(), 2, 1, &chosen, &ms) != GSPH_OK) {
+            std::fprintf(stderr, "%s\n", gsph_last_error(system.context()));
+            return EXIT_FAILURE;
+        }
+        std::printf("Autotuned: %d blocks per CU, staging depth %d, pair masks %d, split %s "
+                    "(%.3f ms per trial step)\n",
+                    chosen.blocks_per_cu, chosen.stage_depth, chosen.pair_masks,
+                    chosen.split_mode == 0 ? "reference" : "tuned", ms);
+    }
     system.runBenchmark(frames);
     return EXIT_SUCCESS;
 }
