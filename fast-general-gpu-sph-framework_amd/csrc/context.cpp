@@ -935,6 +935,99 @@ int gsph_set_state(gsph_ctx *ctx, const float *pos4, const float *vel3, const fl
     return GSPH_OK;
 }
 
+namespace {
+
+// Reallocates every buffer for cap particles and copies the resident state
+// (pos, vel, ev, acc of slots [0, n)).  On failure the context keeps its old
+// buffers untouched.
+int grow_particles(gsph_ctx *ctx, int cap) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const gsph_ctx saved = *ctx;   // host copy: pointers, sizes, bookkeeping
+    ctx->allocs.clear();
+    int rc = alloc_buffers(ctx, cap, saved.cap_c, saved.cap_s);
+    const size_t n = (size_t)saved.n;
+    hipStream_t s = ctx->stream;
+    if (!rc && n) {
+        hipError_t e = hipMemcpyAsync(ctx->pos[0], saved.pos[0], 16 * n, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ctx->vel[0], saved.vel[0], 12 * n, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ctx->ev[0], saved.ev[0], 16 * n, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ctx->acc, saved.acc, 12 * n, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(ctx, GSPH_ERR_HIP, std::string("grow: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(s);
+        for (void *p : ctx->allocs) (void)hipFree(p);
+        const std::string err = ctx->err;
+        *ctx = saved;
+        ctx->err = err;
+        return rc;
+    }
+    for (void *p : saved.allocs) (void)hipFree(p);
+    return GSPH_OK;
+}
+
+}  // namespace
+
+int gsph_add_particles(gsph_ctx *ctx, int count, const float *pos4, const float *vel3,
+                       int reset_all) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    if (ctx->batch)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "particles are added to plain contexts only");
+    if (count < 0 || (count > 0 && !pos4))
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "bad particle count or null pos4");
+    if (ctx->cur != 0)
+        return fail(ctx, GSPH_ERR_INVALID_ARG,
+                    "particles are added between steps (call integrate first)");
+    constexpr long long kMaxParticles = 0x7fffff00LL;
+    const long long n_new = (long long)ctx->n + count;
+    if (n_new > kMaxParticles) return fail(ctx, GSPH_ERR_CAPACITY, "too many particles");
+    const auto &w = ctx->worlds[0];
+    std::vector<float> p4((size_t)count * 4);
+    for (int k = 0; k < count; ++k) {
+        for (int a = 0; a < 3; ++a) {
+            const float x = pos4[4 * (size_t)k + a];
+            if (!std::isfinite(x) || x < 0.f || x >= w[a])
+                return fail(ctx, GSPH_ERR_OUT_OF_DOMAIN,
+                            "particle " + std::to_string(k) + " is outside the box");
+            p4[4 * (size_t)k + a] = x;
+        }
+        p4[4 * (size_t)k + 3] = 0.f;
+    }
+    // errors raised by earlier steps are reported before the counters are
+    // replaced
+    int rc = collect_errors(ctx);
+    if (rc) return rc;
+    if (n_new > ctx->cap_n) {
+        const long long cap = std::min(kMaxParticles, std::max(n_new, 2LL * ctx->cap_n));
+        rc = grow_particles(ctx, (int)cap);
+        if (rc) return rc;
+    }
+    const size_t n = (size_t)ctx->n, c = (size_t)count;
+    hipStream_t s = ctx->stream;
+    if (c) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->pos[0] + n, p4.data(), 16 * c, hipMemcpyHostToDevice, s));
+        if (vel3)
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->vel[0] + n, vel3, 12 * c, hipMemcpyHostToDevice, s));
+        else
+            HIP_TRY(ctx, hipMemsetAsync(ctx->vel[0] + n, 0, 12 * c, s));
+        launch_init_state(ctx->ev[0] + n, ctx->acc + n, count, s);
+    }
+    if (reset_all && n_new > 0) launch_init_state(ctx->ev[0], ctx->acc, (int)n_new, s);
+    rc = check_launch(ctx, "init_state");
+    if (rc) return rc;
+    SceneGeom g = ctx->scenes[0];
+    g.p_count = (int)n_new;
+    rc = configure_layout(ctx, {g}, (int)n_new);
+    if (!rc) rc = invalidate_order(ctx);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return GSPH_OK;
+}
+
 int gsph_arrange(gsph_ctx *ctx) {
     if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
     int rc = consume_injection(ctx);

@@ -55,8 +55,38 @@ HybridSystem::HybridSystem(const float3h &real_world_side, const float3h &sim_or
 
 HybridSystem::~HybridSystem() { gsph_destroy(ctx_); }
 
+// HybridSystem::insertParticles (sph_hybrid_system.cpp:576-582).
+void HybridSystem::insertParticles(unsigned type) {
+    if (type == 1) action1_ = !action1_;
+}
+
+// The emitter of sph_hybrid_system.cpp:653-676: a block x, z in [0.48, 0.52),
+// y in [0.6, 0.7) moving at (0, -6, 0), with ev and acc of every particle
+// reset as its BuffInit over all particles does.  The reference steps the
+// lattice by spacing_fluid, which it never sets (line 196 is commented out);
+// here the step is h/2, the lattice spacing of the scene files.
+void HybridSystem::action1() {
+    const float lo[3] = {0.48f, 0.6f, 0.48f}, hi[3] = {0.52f, 0.7f, 0.52f};
+    const float step = 0.5f * params_.kernel_h;
+    std::vector<float> pos, vel;
+    for (float x = lo[0]; x < hi[0]; x += step)
+        for (float y = lo[1]; y < hi[1]; y += step)
+            for (float z = lo[2]; z < hi[2]; z += step) {
+                pos.insert(pos.end(), {x, y, z, 0.f});
+                vel.insert(vel.end(), {0.f, -6.f, 0.f});
+            }
+    const int count = (int)(pos.size() / 4);
+    if (gsph_add_particles(ctx_, count, pos.data(), vel.data(), 1) != GSPH_OK) {
+        std::cerr << "insert: " << gsph_last_error(ctx_) << std::endl;
+        action1_ = false;
+        return;
+    }
+    nump_ += (unsigned)count;
+}
+
 void HybridSystem::tick() {
     if (!is_running_ || !ctx_) return;
+    if (action1_) action1();
     if (gsph_step(ctx_, 1, &last_) != GSPH_OK)
         std::cerr << "tick: " << gsph_last_error(ctx_) << std::endl;
 }

@@ -38,12 +38,18 @@ public:
     bool isRunning();
     unsigned getNumParticles();
     float3h getPosition(unsigned idx);
+    // type 1 toggles the particle emitter: while it is on, every tick first
+    // appends one emitter block (action1) through gsph_add_particles
+    void insertParticles(unsigned type);
 
     bool ok() const { return ctx_ != nullptr; }
     gsph_ctx *context() { return ctx_; }
 
 private:
+    void action1();
+
     gsph_ctx *ctx_ = nullptr;
+    bool action1_ = false;
     gsph_params params_{};
     unsigned nump_ = 0;
     bool is_running_ = false;

@@ -137,6 +137,7 @@ _SIGS = {
     "gsph_num_particles": (_I, [_P, C.POINTER(_I)]),
     "gsph_get_constants": (_I, [_P, C.POINTER(Constants)]),
     "gsph_set_state": (_I, [_P, _P, _P, _P]),
+    "gsph_add_particles": (_I, [_P, _I, _P, _P, _I]),
     "gsph_arrange": (_I, [_P]),
     "gsph_density": (_I, [_P]),
     "gsph_force": (_I, [_P]),
@@ -327,6 +328,17 @@ class System:
         vel3 = None if vel3 is None else np.ascontiguousarray(vel3, np.float32)
         ev4 = None if ev4 is None else np.ascontiguousarray(ev4, np.float32)
         self._c(self._lib.gsph_set_state(self.h, _ptr(pos4), _ptr(vel3), _ptr(ev4)))
+
+    def add_particles(self, pos4, vel3=None, reset_all=False):
+        """Append particles between steps (plain contexts; w of pos4 is ignored,
+        ev starts at 0); reset_all also zeroes ev of the particles already there."""
+        pos4 = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        vel3 = None if vel3 is None else np.ascontiguousarray(vel3, np.float32).reshape(-1, 3)
+        if vel3 is not None and vel3.shape[0] != pos4.shape[0]:
+            raise ValueError("vel3 and pos4 differ in length")
+        self._c(self._lib.gsph_add_particles(self.h, pos4.shape[0], _ptr(pos4), _ptr(vel3),
+                                             1 if reset_all else 0))
+        self.n += pos4.shape[0]
 
     def arrange(self):
         self._c(self._lib.gsph_arrange(self.h))

@@ -243,6 +243,21 @@ int gsph_get_constants(const gsph_ctx *ctx, gsph_constants *out);
 /* Overwrites the device state (any of vel3/ev4 may be NULL = zero). */
 int gsph_set_state(gsph_ctx *ctx, const float *pos4, const float *vel3,
                    const float *ev4);
+/* HybridSystem::addParticle (src/simulation/sph_hybrid_system.cpp:602-627)
+ * with Arrangement::resetNumParticle (src/grid/sph_arrangement.cu:1643-1670):
+ * appends count particles to a plain context between steps.  pos4 gives x, y,
+ * z (w is ignored and stored as 0, as addParticle does), vel3 may be NULL
+ * (zero velocity), ev and acc start at 0.  Positions must be finite and
+ * inside the box [0, world).  When the particle capacity is exceeded every
+ * per-particle buffer grows to max(2 x capacity, n + count), as the
+ * reference's buffers double, and the state is kept; unlike the reference,
+ * the sort, split and task buffers grow with it.  reset_all != 0 also zeroes
+ * ev and acc of the particles already there, as the reference's emitter
+ * (action1, sph_hybrid_system.cpp:653-676) does with BuffInit over all
+ * particles.  The next arrange sorts every particle; a captured step graph is
+ * dropped. */
+int gsph_add_particles(gsph_ctx *ctx, int count, const float *pos4, const float *vel3,
+                       int reset_all);
 /* Arrangement::arrangeHybridFrame (src/grid/sph_arrangement.cu:1493-1506):
  * hash, stable sort, permute, cell begin/count, split, compact index, tasks,
  * pairing.  No host synchronisation. */

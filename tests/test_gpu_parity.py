@@ -270,6 +270,27 @@ def test_launch_shape_changes_nothing(gsph, po, name, opts, depth, bpc):
         assert np.array_equal(out[0][k], out[1][k])
 
 
+@pytest.mark.parametrize("name", ["mixed", "dense"])
+def test_split_changes_no_bits(gsph, po, name):
+    """The SMS/TRA split moves no bit: both paths walk the same nine rows in
+    the same (dy, dz) order with candidates in index order, density sums each
+    row apart before adding it, and a rejected candidate adds an exact zero
+    (an SMS task's wider x-range only adds rejected candidates)."""
+    op, pos = scene(po, name)
+    banner(f"split neutrality {name}")
+    out, tasks = [], []
+    for o in ({}, {"split_mode": 1, "sms_min_self": 9, "sms_min_total": 64}, ALL_TRA):
+        s = make_system(gsph, op, pos, **o)
+        s.step(3)
+        out.append(s.read_state())
+        tasks.append(s.grid_info()["tasks"])
+        s.close()
+    assert tasks[0] > 0 and tasks[2] == 0, tasks
+    for j in (1, 2):
+        for k in range(4):
+            assert np.array_equal(out[0][k], out[j][k]), (j, k)
+
+
 def test_autotune_restores_state_and_keeps_bits(gsph, po):
     op, pos = scene(po, "mixed")
     banner("autotune mixed")
