@@ -570,7 +570,10 @@ int orc_density_brute_f64(const orc_params *p, int n, const float *pos4,
 // part), plus the surface force and its per-neighbour contributions.
 int orc_force_f64(const orc_params *p, int n, const float *pos4,
                   const float *ev4, const double *rho, const double *pressure,
-                  double *acc, double *acc_scale) {
+                  double *acc, double *acc_scale, double *acc_alt) {
+    // the relative bar of the physics parity: an FP32 grad c (FP32 1/rho_j,
+    // a few roundings per term and the sum) is within it of the FP64 one
+    constexpr double kBranchTol = 1e-5;
     orc_constants c;
     orc_derive(p, &c);
     const DConst d = dconst(p);
@@ -615,6 +618,16 @@ int orc_force_f64(const orc_params *p, int n, const float *pos4,
         if (gn > d.surface_normal)
             for (int a = 0; a < 3; ++a) S[a] = G[a] * d.surface_coe * L / gn;
         for (int a = 0; a < 3; ++a) acc[3 * i + a] = F[a] * d.mass + S[a];
+        if (acc_alt) {
+            const double margin = kBranchTol * std::fabs(d.grad_scale) * Gs;
+            const bool near = std::fabs(gn - d.surface_normal) <= margin && gn > 0;
+            for (int a = 0; a < 3; ++a) {
+                // the other branch: surface force off if it was on, and on if off
+                const double s_alt =
+                    gn > d.surface_normal ? 0.0 : G[a] * d.surface_coe * L / gn;
+                acc_alt[3 * i + a] = near ? F[a] * d.mass + s_alt : acc[3 * i + a];
+            }
+        }
         if (acc_scale) {
             // surface force G coe L / |G|: its per-neighbour contributions are
             // coe |L_j| (the Laplacian sum cancels between r < 0.655 h and

@@ -79,9 +79,14 @@ int orc_density_f64(const orc_params *p, int n, const float *pos4, double *rho,
                     double *pressure);
 int orc_density_brute_f64(const orc_params *p, int n, const float *pos4,
                           double *rho);
+/* acc_alt (nullable): the surface force switches on at |grad c| >
+ * surface_normal, a discontinuity.  Where |grad c| lies within
+ * kBranchTol * sum_j |grad c terms| of the threshold either branch is a correct
+ * result of an FP32 evaluation (it rounds grad c by about that much), and
+ * acc_alt holds the other branch's acceleration; elsewhere acc_alt = acc. */
 int orc_force_f64(const orc_params *p, int n, const float *pos4,
                   const float *ev4, const double *rho, const double *pressure,
-                  double *acc, double *acc_scale);
+                  double *acc, double *acc_scale, double *acc_alt);
 int orc_integrate_f64(const orc_params *p, int n, const float *pos4,
                       const float *vel3, const float *ev4, const double *acc,
                       const double *rho, double *pos_out, double *vel_out,

@@ -59,7 +59,7 @@ _SIGS = {
     "orc_arrange": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I]),
     "orc_density_f64": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P]),
     "orc_density_brute_f64": (_I, [C.POINTER(OrcParams), _I, _P, _P]),
-    "orc_force_f64": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P, _P, _P, _P]),
+    "orc_force_f64": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P, _P, _P, _P, _P]),
     "orc_integrate_f64": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "orc_step_f32": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P, _P, _I, _I]),
     "orc_checksum": (_I, [_I, _P, _P]),
@@ -181,15 +181,19 @@ def density_brute_f64(p, pos4):
     return rho
 
 
-def force_f64(p, pos4, ev4, rho, pressure):
+def force_f64(p, pos4, ev4, rho, pressure, alt=False):
+    """FP64 acceleration and its scale; with alt=True also the acceleration of
+    the other surface-force branch (equal to acc except where |grad c| is
+    within FP32 rounding of the threshold)."""
     pos4 = np.ascontiguousarray(pos4, np.float32)
     ev4 = np.ascontiguousarray(ev4, np.float32)
     n = pos4.shape[0]
     acc = np.zeros((n, 3))
     scale = np.zeros(n)
+    acc_alt = np.zeros((n, 3)) if alt else None
     load().orc_force_f64(C.byref(p), n, _p(pos4), _p(ev4), _p(np.ascontiguousarray(rho)),
-                         _p(np.ascontiguousarray(pressure)), _p(acc), _p(scale))
-    return acc, scale
+                         _p(np.ascontiguousarray(pressure)), _p(acc), _p(scale), _p(acc_alt))
+    return (acc, scale, acc_alt) if alt else (acc, scale)
 
 
 def integrate_f64(p, pos4, vel3, ev4, acc, rho):
@@ -242,8 +246,14 @@ def fp64_step(p, pos4, vel3, ev4):
     ev_f[:, 3] = pr.astype(np.float32)
     pos_f = pos_s.copy()
     pos_f[:, 3] = (1.0 / rho).astype(np.float32)
-    acc, scale = force_f64(p, pos_f, ev_s, rho, pr)
+    acc, scale, acc_alt = force_f64(p, pos_f, ev_s, rho, pr, alt=True)
     po, vo, eo = integrate_f64(p, pos_s, vel_s, ev_s, acc, rho)
     eo[:, 3] = pr
+    # particles on the surface-force threshold: the other branch's outcome
+    branch = np.any(acc_alt != acc, axis=1)
+    pa, va = po, vo
+    if branch.any():
+        pa, va, _ = integrate_f64(p, pos_s, vel_s, ev_s, acc_alt, rho)
     return dict(grid=g, pos_sorted=pos_s, vel_sorted=vel_s, ev_sorted=ev_s, rho=rho,
-                pressure=pr, acc=acc, acc_scale=scale, pos=po, vel=vo, ev=eo)
+                pressure=pr, acc=acc, acc_scale=scale, pos=po, vel=vo, ev=eo,
+                branch=branch, acc_alt=acc_alt, pos_alt=pa, vel_alt=va)

@@ -517,3 +517,40 @@ def test_config1_full_size_properties(gsph, po):
     assert np.array_equal(g["perm"], np.lexsort((np.arange(n), key)))
     assert np.array_equal(g["hash"], key[g["perm"]])
     s.close()
+
+
+def test_config2_full_size(gsph, po):
+    """Workload 2 at full size (8,137,432 particles: a 0.01-spaced pool and a
+    0.005-spaced sphere, 0 to 64 per cell): arrange bit-exact against the
+    oracle under the reference split rule, one step within tolerance of the
+    FP64 oracle, and the autotuned split giving the same bits."""
+    import pyoracle
+    p, pos = gsph.scene_config(2)
+    n = len(pos)
+    banner(f"config2 n={n}")
+    op = pyoracle.params_from(p)
+    s = gsph.System(p, pos)
+    s.arrange()
+    g = s.read_grid()
+    ref = po.arrange(op, pos)
+    compare_grid(g, ref)
+    assert ref["tasks_count"] > 100000 and 0 < ref["middle_raw"] < n
+    s.density()
+    s.force()
+    s.integrate()
+    pos1, vel1, ev1, acc1 = s.read_state()
+    s.close()
+    ref = po.fp64_step(op, pos, np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32))
+    errs = step_errors(pos1, vel1, ev1, acc1, ref, op)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e}"
+    t = gsph.System(p, pos)
+    opt, ms = t.autotune(trial_steps=1, tune_split=True)
+    print("chosen", opt.split_mode, opt.sms_min_self, opt.sms_min_total, opt.blocks_per_cu,
+          opt.stage_depth, opt.pair_masks, f"{ms:.3f} ms", flush=True)
+    t.step(1)
+    got = t.read_state()
+    t.close()
+    for k, a in enumerate((pos1, vel1, ev1, acc1)):
+        assert np.array_equal(got[k], a), k

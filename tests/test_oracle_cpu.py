@@ -312,6 +312,38 @@ def test_force_two_particle_closed_form(po):
     assert acc[0, 2] == 0.0
 
 
+@pytest.mark.parametrize("delta,near", [(1e-7, True), (-1e-7, True), (1e-3, False), (-1e-3, False)])
+def test_force_branch_alternative_on_surface_threshold(po, delta, near):
+    """Two particles, rho_1 chosen so that particle 0's |grad c| is
+    (1 + delta) x surface_normal: within FP32 reach of the threshold the
+    oracle also returns the other branch (surface force toggled), elsewhere
+    the same acceleration."""
+    p = make_params(h=0.02, mass=1e-3, world=(0.2, 0.2, 0.2))
+    h = float(np.float32(0.02))
+    m = float(np.float32(1e-3))
+    pos = np.array([[0.1, 0.1, 0.1, 0], [0.11, 0.1, 0.1, 0]], np.float32)
+    r = float(np.float32(0.11)) - float(np.float32(0.1))
+    t = h * h - r * r
+    gscale = 945 / (32 * math.pi * h ** 9) * m
+    thr = float(np.float32(0.1))
+    V = thr * (1 + delta) / (r * t * t * gscale)
+    rho = np.array([900.0, 1 / V])
+    pr = np.array([0.3, -0.2])
+    acc, scale, alt = po.force_f64(p, pos, np.zeros_like(pos), rho, pr, alt=True)
+    kr = h - r
+    base = m * (r * V * (pr[0] + pr[1]) * kr * kr / r) * (-45 / (math.pi * h ** 6) / 2)
+    G = r * V * t * t * gscale            # rel = -r and grad_poly6 < 0: G_x > 0
+    L = V * t * (r * r - 0.75 * t) * (945 / (8 * math.pi * h ** 9)) * m / rho[0]
+    surf = float(np.float32(0.2)) * L     # G coe L / |G| along +x
+    on = delta > 0
+    assert acc[0, 0] == pytest.approx(base + (surf if on else 0.0), rel=1e-9)
+    other = base + (0.0 if on else surf)
+    assert alt[0, 0] == pytest.approx(other if near else acc[0, 0], rel=1e-9)
+    assert abs(surf) > 1e-3 * abs(base)   # the two branches differ visibly
+    assert np.array_equal(alt[1], acc[1])  # |grad c| of particle 1 is 0.18 x threshold
+    assert np.array_equal(alt[0, 1:], acc[0, 1:])
+
+
 def test_force_symmetric_lattice_interior_is_balanced(po):
     # dyadic spacing and corner: every lattice coordinate is exact in float32
     s = 2.0 ** -7

@@ -10,6 +10,12 @@ proportional to:
                magnitudes (pressure terms taken with the uncancelled p)
   position     |x - x*| / |x*|
   velocity     |v - v*| / max(|v*|, dt (w* S + |g|))
+
+The surface force is discontinuous: it switches on at |grad c| =
+surface_normal.  A particle whose FP64 |grad c| lies within the bar of that
+threshold (the oracle's `branch` mask) may take either side in FP32, so its
+acceleration, velocity and position are measured against the nearer of the
+two outcomes the oracle returns.
 """
 import numpy as np
 
@@ -30,11 +36,21 @@ def step_errors(pos1, vel1, ev1, acc1, ref, op):
     a_ref = ref["acc"]
     ascale = np.maximum(np.linalg.norm(a_ref, axis=1), ref["acc_scale"])
     ascale = np.maximum(ascale, 1e-30)
-    out["acceleration"] = float(np.max(np.linalg.norm(acc1 - a_ref, axis=1) / ascale))
+    branch = ref.get("branch")
+
+    def nearer(got, key):
+        """Per-particle distance to the oracle's outcome, or to the nearer of
+        its two outcomes on the surface-force threshold."""
+        d = np.linalg.norm(got - ref[key][:, :got.shape[1]], axis=1)
+        if branch is not None and branch.any():
+            alt = np.linalg.norm(got - ref[key + "_alt"][:, :got.shape[1]], axis=1)
+            d = np.where(branch, np.minimum(d, alt), d)
+        return d
+
+    out["acceleration"] = float(np.max(nearer(acc1, "acc") / ascale))
     x_ref = ref["pos"][:, :3]
-    out["position"] = float(np.max(np.linalg.norm(pos1[:, :3] - x_ref, axis=1) /
-                                   np.linalg.norm(x_ref, axis=1)))
+    out["position"] = float(np.max(nearer(pos1[:, :3], "pos") / np.linalg.norm(x_ref, axis=1)))
     v_ref = ref["vel"]
     vscale = np.maximum(np.linalg.norm(v_ref, axis=1), dt * (ref["acc_scale"] / rho + g))
-    out["velocity"] = float(np.max(np.linalg.norm(vel1 - v_ref, axis=1) / vscale))
+    out["velocity"] = float(np.max(nearer(vel1, "vel") / vscale))
     return out
