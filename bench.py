@@ -73,30 +73,11 @@ def opt_dict(o):
     return {k: getattr(o, k) for k, _ in o._fields_}
 
 
-def batch_scenes(gsph, count=256, seed=20261206):
-    """Workload 3: `count` dam-break blocks, N log-uniform in [4096, 262144],
-    aspect ratios y/x and z/x uniform in [0.5, 2], spacing 0.01, corner 0.06,
-    box 2x the block extent per axis."""
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    worlds, blocks = [], []
-    for k in range(count):
-        n_target = float(np.exp(rng.uniform(np.log(4096), np.log(262144))))
-        ay, az = rng.uniform(0.5, 2.0, 2)
-        nx = max(4, int(round((n_target / (ay * az)) ** (1 / 3))))
-        ny, nz = max(4, int(round(ay * nx))), max(4, int(round(az * nx)))
-        pos = gsph.scene_block([0.06, 0.06, 0.06], nx, ny, nz, 0.01, 0.01, seed + 7919 * (k + 1))
-        worlds.append([2 * nx * 0.01, 2 * ny * 0.01, 2 * nz * 0.01])
-        blocks.append(pos)
-    return worlds, blocks
-
-
 def batched_leg(gsph, steps, warmup, count=256, loop_steps=5, no_autotune=False):
     import numpy as np
 
     params, _ = gsph.scene_config(0)
-    worlds, blocks = batch_scenes(gsph, count)
+    worlds, blocks = gsph.batch_workload(count)
     counts = [len(b) for b in blocks]
     n = int(sum(counts))
     b = gsph.System(params, np.concatenate(blocks), batch_worlds=worlds, batch_counts=counts)

@@ -263,6 +263,23 @@ def scene_block(corner, nx, ny, nz, spacing, jitter_frac, seed):
     return pos
 
 
+def batch_workload(count=256, seed=20261206):
+    """The batched dam-break workload: `count` lattice blocks (spacing 0.01,
+    jitter 1%, corner 0.06) with N log-uniform in [4096, 262144] and aspect
+    ratios y/x, z/x uniform in [0.5, 2], each in a box 2x its extent.
+    Returns (worlds, blocks)."""
+    rng = np.random.default_rng(seed)
+    worlds, blocks = [], []
+    for k in range(count):
+        n_target = float(np.exp(rng.uniform(np.log(4096), np.log(262144))))
+        ay, az = rng.uniform(0.5, 2.0, 2)
+        nx = max(4, int(round((n_target / (ay * az)) ** (1 / 3))))
+        ny, nz = max(4, int(round(ay * nx))), max(4, int(round(az * nx)))
+        blocks.append(scene_block([0.06, 0.06, 0.06], nx, ny, nz, 0.01, 0.01, seed + 7919 * (k + 1)))
+        worlds.append([2 * nx * 0.01, 2 * ny * 0.01, 2 * nz * 0.01])
+    return worlds, blocks
+
+
 class System:
     """One context: a plain scene (gsph_create) or a batch (gsph_batch_create)."""
 

@@ -480,8 +480,10 @@ def test_batch_alone_vs_mix(gsph, po):
 
 
 def test_config1_full_size_properties(gsph, po):
-    """Workload 1 at full size (3,944,312 particles, 317^3 cells): properties
-    that do not depend on the oracle fitting in memory."""
+    """Workload 1 at full size (3,944,312 particles, 317^3 cells): the sort
+    against the oracle's keys, the first step against the FP64 oracle, the
+    checksum after 10 steps against the FP32 CPU step, and the sort by
+    merging on later steps."""
     p, pos = gsph.scene_config(1)
     n = len(pos)
     banner(f"config1 n={n}")
@@ -503,9 +505,21 @@ def test_config1_full_size_properties(gsph, po):
     s.density()
     s.force()
     s.integrate()
-    s.sync()
+    pos1, vel1, ev1, acc1 = s.read_state()
+    ref = po.fp64_step(pyoracle.params_from(p), pos, np.zeros((n, 3), np.float32),
+                       np.zeros((n, 4), np.float32))
+    errs = step_errors(pos1, vel1, ev1, acc1, ref, pyoracle.params_from(p))
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e}"
+    del ref
+    # ten steps: the checksum against the FP32 CPU step of the reference
+    s.step(9)
     cs = s.checksum()
-    assert np.all(np.isfinite(cs))
+    P, _, _, _ = po.step_f32(pyoracle.params_from(p), pos, nsteps=10)
+    rel = np.abs(cs - po.checksum(P)) / np.abs(po.checksum(P))
+    print("checksum rel diff", rel, flush=True)
+    assert rel.max() <= CHECKSUM_TOL
     # later steps sort by merging the moved particles into the previous
     # order: still the stable sort of the keys of the state in slot order
     s.step(3)
@@ -554,3 +568,50 @@ def test_config2_full_size(gsph, po):
     t.close()
     for k, a in enumerate((pos1, vel1, ev1, acc1)):
         assert np.array_equal(got[k], a), k
+
+
+def test_config3_full_size_batch(gsph, po):
+    """Workload 3 at full size (256 scenes, 17,280,876 particles, autotuned,
+    hipGraph step): sampled scenes give bitwise what they give alone, and the
+    first step of two of them is within tolerance of the FP64 oracle."""
+    import pyoracle
+    worlds, blocks = gsph.batch_workload(256)
+    counts = [len(b) for b in blocks]
+    banner(f"config3 scenes=256 n={sum(counts)}")
+    params, _ = gsph.scene_config(0)
+    b = gsph.System(params, np.concatenate(blocks), batch_worlds=worlds, batch_counts=counts)
+    b.autotune(trial_steps=1)
+    b.enable_graph(True)
+    b.step(1)
+    first = b.read_state()
+    b.step(2)
+    state = b.read_state()
+    order = np.argsort(counts)
+    rng = np.random.default_rng(3)
+    sample = [int(order[0]), int(order[-1])] + [int(k) for k in rng.choice(256, 4, replace=False)]
+    for k in sample:
+        p, _ = gsph.scene_config(0)
+        for a in range(3):
+            p.world_size[a] = worlds[k][a]
+        s = gsph.System(p, blocks[k])
+        s.step(3)
+        alone = s.read_state()
+        f, cnt = b.scene_range(k)
+        assert cnt == counts[k]
+        for x, y in zip(state, alone):
+            assert np.array_equal(x[f:f + cnt], y), k
+        assert np.array_equal(b.checksum(f, cnt), s.checksum())
+        s.close()
+    for k in sample[:2]:
+        p, _ = gsph.scene_config(0)
+        for a in range(3):
+            p.world_size[a] = worlds[k][a]
+        op = pyoracle.params_from(p)
+        f, cnt = b.scene_range(k)
+        ref = po.fp64_step(op, blocks[k], np.zeros((cnt, 3), np.float32),
+                           np.zeros((cnt, 4), np.float32))
+        errs = step_errors(*(x[f:f + cnt] for x in first), ref, op)
+        print(k, cnt, {n: f"{v:.2e}" for n, v in errs.items()}, flush=True)
+        for n, v in errs.items():
+            assert v <= REL_TOL, f"scene {k} {n} error {v:.3e}"
+    b.close()
