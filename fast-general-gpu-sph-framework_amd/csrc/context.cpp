@@ -270,6 +270,7 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.dc = ctx->dc;
     p.grid_blocks = ctx->grid_blocks;
     p.row_prune = ctx->opt.row_prune;
+    p.sms_local_merge = ctx->opt.sms_local_merge;
     p.nbmask = ctx->nbmask;
     p.mask_stride = ctx->cap_n;
     p.use_masks = 0;
@@ -533,6 +534,8 @@ int check_options(gsph_ctx *ctx, gsph_options &o) {
         return fail(ctx, GSPH_ERR_INVALID_ARG, "split_mode must be 0 or 1");
     if (o.sms_mask_bytes < 0 || o.sms_mask_bytes > 65536)
         return fail(ctx, GSPH_ERR_INVALID_ARG, "sms_mask_bytes must be in [0, 65536]");
+    if (o.sms_local_merge != 0 && o.sms_local_merge != 1)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "sms_local_merge must be 0 or 1");
     return GSPH_OK;
 }
 
@@ -1223,6 +1226,11 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
         if (best.split_mode != shape.split_mode || best.sms_min_self != shape.sms_min_self ||
             best.sms_min_total != shape.sms_min_total)
             shape_search(best);
+    }
+    {   // shared staging of isSame pairs on / off (no effect without SMS tasks)
+        gsph_options o = best;
+        o.sms_local_merge ^= 1;
+        trial(o);
     }
     (void)hipFree(pos0);
     (void)hipFree(vel0);

@@ -95,6 +95,7 @@ struct PhysicsArgs {
     const int *dc;
     int grid_blocks;       // persistent grid of density / force
     int row_prune;
+    int sms_local_merge;   // isSame pairs of one cell stage their rows once
     // accepted-pair masks and row plans of TRA particles: kMaskWords words per
     // entry of compact, word-major with stride mask_stride; written by
     // density, read by force

@@ -257,7 +257,7 @@ __device__ __forceinline__ SmsItem sms_item(const PhysicsArgs &P, int t0, int T,
     m.cy = m.has_task ? tk.cell_pos[1] : 0;
     m.cz = m.has_task ? tk.cell_pos[2] : 0;
     int xi = m.has_task ? tk.xxi : 0, xa = m.has_task ? tk.xxx : 0;
-    m.shared = t0 + 1 < T && P.tasks[t0].cellid == P.tasks[t0 + 1].cellid;
+    m.shared = P.sms_local_merge && t0 + 1 < T && P.tasks[t0].cellid == P.tasks[t0 + 1].cellid;
     if (m.shared) {
         xi = min((int)P.tasks[t0].xxi, (int)P.tasks[t0 + 1].xxi);
         xa = max((int)P.tasks[t0].xxx, (int)P.tasks[t0 + 1].xxx);

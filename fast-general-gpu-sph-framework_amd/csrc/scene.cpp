@@ -222,6 +222,7 @@ void gsph_options_default(gsph_options *out) {
     out->row_prune = 1;
     out->stage_depth = 128;
     out->sms_mask_bytes = 1024;
+    out->sms_local_merge = 1;
 }
 
 int gsph_derive_constants(const gsph_params *p, gsph_constants *c) {

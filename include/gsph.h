@@ -118,8 +118,8 @@ typedef struct gsph_stage_ms {
  * thresholds below (same physics, different TRA/SMS assignment).  The
  * remaining fields are launch-shape knobs; gsph_autotune picks them by
  * timing the context's own scene.  Only split_mode/thresholds change which
- * path a particle takes (results move within FP32 rounding); the others give
- * bit-identical results. */
+ * path a particle takes; every option gives bit-identical results (the TRA
+ * and SMS paths sum each particle's neighbours in the same order). */
 typedef struct gsph_options {
     int split_mode;
     int sms_min_self;     /* tuned: cell is TRA when n <  sms_min_self ... */
@@ -130,6 +130,10 @@ typedef struct gsph_options {
     int stage_depth;      /* LDS staging depth of TRA neighbour rows: 64, 128, 256 */
     int sms_mask_bytes;   /* SMS accepted-pair mask pool, bytes per particle of
                              capacity (set at creation; 0 = none) */
+    int sms_local_merge;  /* 1: the two tasks of a judgeTask pair in one cell
+                             stage their shared rows once, in density and force
+                             (GSPH_ENABLE_SMS_LOCAL_MERGE and ..._FORCE,
+                             CMakeLists.txt:76-91); 0: each task stages its own */
 } gsph_options;
 
 typedef struct gsph_ctx gsph_ctx;

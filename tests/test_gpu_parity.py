@@ -36,6 +36,11 @@ def scene(po, name):
         return dense_scene(po)
     if name == "mixed":
         return mixed_scene(po)
+    if name == "dense64":
+        # cells of exactly 64 (lattice aligned with the grid): every task
+        # pair is a full isSame pair of one cell
+        p = make_params(h=0.02, world=(0.24, 0.24, 0.24))
+        return p, po.scene_block([0.06, 0.06, 0.06], 16, 16, 16, 0.005, 0.01, 1)
     if name == "odd_tasks":
         # one cell row of dense cells whose task count is odd
         # a dense block (20 tasks) plus one isolated cell of 20 particles
@@ -55,7 +60,7 @@ def scene(po, name):
     raise KeyError(name)
 
 
-SCENES = ["cfg0", "dense", "mixed", "odd_tasks", "sparse_random"]
+SCENES = ["cfg0", "dense", "dense64", "mixed", "odd_tasks", "sparse_random"]
 
 
 def make_system(gsph, op, pos, **opts):
@@ -90,8 +95,10 @@ def test_arrange_bit_exact(gsph, po, name):
     g = s.read_grid()
     ref = po.arrange(op, pos)
     compare_grid(g, ref)
-    if name in ("dense", "mixed", "odd_tasks"):
+    if name in ("dense", "dense64", "mixed", "odd_tasks"):
         assert ref["tasks_count"] > 0
+    if name == "dense64":
+        assert s.pair_stats()["same_cell_pairs"] == ref["tasks_count"] // 2
     if name == "odd_tasks":
         assert ref["tasks_count"] & 1
     s.close()
@@ -268,6 +275,24 @@ def test_launch_shape_changes_nothing(gsph, po, name, opts, depth, bpc):
         s.close()
     for k in range(4):
         assert np.array_equal(out[0][k], out[1][k])
+
+
+@pytest.mark.parametrize("name", ["mixed", "dense64", "odd_tasks"])
+def test_sms_local_merge_changes_no_bits(gsph, po, name):
+    """Shared staging of an isSame pair's rows (the reference's SMS local
+    merge) on and off: the staged set changes, each lane's pairs and their
+    order do not."""
+    op, pos = scene(po, name)
+    banner(f"sms local merge {name}")
+    out = []
+    for merge in (1, 0):
+        s = make_system(gsph, op, pos, sms_local_merge=merge)
+        s.step(3)
+        out.append(s.read_state())
+        assert s.pair_stats()["same_cell_pairs"] > 0   # the shared staging runs
+        s.close()
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k]), k
 
 
 @pytest.mark.parametrize("name", ["mixed", "dense"])
