@@ -84,18 +84,6 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Mask of the active lanes whose 9-bit value v equals this lane's.
-__device__ __forceinline__ uint64_t match9(uint32_t v) {
-    uint64_t peers = __ballot(1);
-#pragma unroll
-    for (int b = 0; b < 9; ++b) {
-        const bool bit = (v >> b) & 1u;
-        const uint64_t m = __ballot(bit);
-        peers &= bit ? m : ~m;
-    }
-    return peers;
-}
-
 // Block-wide exclusive scan of one uint32 per thread (blockDim.x = NT, a
 // multiple of 64).  Returns the exclusive prefix, writes the block total.
 template <int NT>
