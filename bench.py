@@ -208,7 +208,15 @@ def main():
                     help="keep the default options (profiling runs: no trial launches)")
     ap.add_argument("--probe-copy", type=float, default=0.0,
                     help="only run a plain copy kernel over this many bytes (counter calibration)")
+    ap.add_argument("--probe-valu", type=int, default=0,
+                    help="only run the FP32 FMA-chain kernel for this many iterations (VALU peak)")
     args = ap.parse_args()
+    if args.probe_valu > 0:
+        import gsph
+        ms, ops = gsph.probe_valu(args.probe_valu)
+        print(json.dumps({"probe_valu_iters": args.probe_valu, "ms": ms, "lane_ops": ops,
+                          "tlane_ops_per_s": ops / (ms * 1e-3) / 1e12}))
+        return
     if args.probe_copy > 0:
         import gsph
         ms = gsph.probe_copy(args.probe_copy, 5)
@@ -261,6 +269,9 @@ def main():
     s.sync()
     info = s.grid_info()
     cs = s.checksum()
+    # the VALU peak of this GPU: independent FP32 FMA chains on 8 waves per SIMD
+    probe_ms, probe_ops = gsph.probe_valu(4000)
+    valu_peak = probe_ops / (probe_ms * 1e-3) / 1e12   # T lane-ops/s
 
     kern = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in kt.items()}
     fused = "integrate" not in kern   # the full step integrates inside the force kernel
@@ -286,6 +297,11 @@ def main():
     traffic = profile_block(TRAFFIC_PROFILE).get("traffic", {})
     valu = profile_block(VALU_PROFILE).get("counters", {})
     dom_traffic = single_scene(traffic, dom).get("bytes")
+    # active-lane VALU operations of one launch of the dominant kernel
+    # (SQ_THREAD_CYCLES_VALU of the committed counter profile) over its live
+    # duration, against the FMA-chain peak measured above
+    dom_lane_ops = single_scene(valu, dom).get("SQ_THREAD_CYCLES_VALU")
+    valu_achieved = dom_lane_ops / (dom_ms * 1e-3) / 1e12 if dom_lane_ops else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -306,8 +322,11 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": dom_traffic, "algorithmic_bytes": alg[dom], "avg_ms": dom_ms,
                      "traffic_source": TRAFFIC_PROFILE.name if dom_traffic else None,
-                     "valu_busy_pct": single_scene(valu, dom).get("valu_busy_pct"),
-                     "valu_util_pct": single_scene(valu, dom).get("valu_util_pct")},
+                     "valu": {"lane_ops": dom_lane_ops, "achieved": valu_achieved,
+                              "peak": valu_peak, "unit": "T lane-ops/s",
+                              "frac": valu_achieved / valu_peak if valu_achieved else None,
+                              "lanes_active_pct": single_scene(valu, dom).get("valu_util_pct"),
+                              "source": VALU_PROFILE.name if dom_lane_ops else None}},
         "step_roofline": {"algorithmic_bytes": alg["step"], "achieved_gbs": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "stages_ms": stage_avg,

@@ -1472,6 +1472,40 @@ int gsph_probe_copy(double bytes, int reps, float *ms) {
     return rc;
 }
 
+int gsph_probe_valu(int iters, float *ms, double *lane_ops) {
+    if (iters <= 0 || iters > (1 << 24)) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe length");
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return fail(nullptr, GSPH_ERR_NO_DEVICE, "no device");
+    const int blocks = 8 * cus;   // 8 blocks of 4 waves per CU: 8 waves per SIMD
+    float *out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = GSPH_OK;
+    if (hipMalloc(&out, sizeof(float) * (size_t)blocks * kProbeThreads) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(nullptr, GSPH_ERR_HIP, "probe allocation failed");
+    } else {
+        launch_fma_probe(out, blocks, 16, s);
+        (void)hipEventRecord(e0, s);
+        launch_fma_probe(out, blocks, iters, s);
+        (void)hipEventRecord(e1, s);
+        if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+            rc = fail(nullptr, GSPH_ERR_HIP, "probe failed");
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (ms) *ms = t;
+        if (lane_ops) *lane_ops = (double)blocks * kProbeThreads * (double)iters * 64.0;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (out) (void)hipFree(out);
+    return rc;
+}
+
 int gsph_inject_fault(gsph_ctx *ctx, int after_calls) {
     if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
     ctx->inject_after = after_calls;

@@ -138,5 +138,8 @@ void launch_checksum(const float4 *pos, int n, double *partial, int nblocks,
                      double *out5, hipStream_t s);
 void launch_init_state(float4 *ev, f3 *acc, int n, hipStream_t s);
 void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s);
+constexpr int kProbeThreads = 256;
+// out: blocks * kProbeThreads floats
+void launch_fma_probe(float *out, int blocks, int iters, hipStream_t s);
 
 }  // namespace gsph

@@ -1036,7 +1036,30 @@ __global__ __launch_bounds__(kNT) void k_copy(const float4 *src, float4 *dst, si
         dst[i] = src[i];
 }
 
+// Eight independent FP32 FMA chains per lane, 64 FMAs per iteration: the
+// peak VALU issue rate (calibration reference for the VALU counters).
+__global__ __launch_bounds__(kProbeThreads) void k_fma_probe(float *out, int iters, float b, float c) {
+    float a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = (float)(threadIdx.x + u);
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = fmaf(a[u], b, c);
+        }
+    }
+    float t = 0.f;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += a[u];
+    out[(size_t)blockIdx.x * kProbeThreads + threadIdx.x] = t;
+}
+
 }  // namespace
+
+void launch_fma_probe(float *out, int blocks, int iters, hipStream_t s) {
+    if (blocks > 0) hipLaunchKernelGGL(k_fma_probe, dim3(blocks), dim3(kProbeThreads), 0, s, out, iters, 0.999f, 0.001f);
+}
 
 void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s) {
     const int blocks = (int)std::min<size_t>((n + kNT - 1) / kNT, 8192);

@@ -159,6 +159,7 @@ _SIGS = {
     "gsph_kernel_times": (_I, [_P, C.c_char_p, _I, _P, _P, _I, C.POINTER(_I)]),
     "gsph_inject_fault": (_I, [_P, _I]),
     "gsph_probe_copy": (_I, [C.c_double, _I, C.POINTER(C.c_float)]),
+    "gsph_probe_valu": (_I, [_I, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
     "gsph_server_create": (_I, [C.POINTER(Params), C.POINTER(Options), C.c_double, _I,
                                 C.POINTER(_P)]),
     "gsph_server_destroy": (_I, [_P]),
@@ -233,6 +234,13 @@ def probe_copy(nbytes: float, reps: int = 10) -> float:
     ms = C.c_float()
     _check(load().gsph_probe_copy(float(nbytes), int(reps), C.byref(ms)))
     return ms.value
+
+
+def probe_valu(iters: int = 4000):
+    """Peak FP32 VALU rate: (ms, lane operations) of one FMA-chain launch."""
+    ms, ops = C.c_float(), C.c_double()
+    _check(load().gsph_probe_valu(int(iters), C.byref(ms), C.byref(ops)))
+    return ms.value, ops.value
 
 
 def scene_config(k: int):

@@ -69,21 +69,33 @@ def cmd_counters(args):
         for k, d in per_dispatch(p).items():
             merged[k].update(d)
     names = sorted({c for d in merged.values() for c in d if c != "dispatches"})
-    print("| kernel | dispatches | " + " | ".join(names) + " | VALU busy | VALU utilisation |")
-    print("|---|---:|" + "---:|" * len(names) + "---:|---:|")
+
+    def lanes_per_cu_cycle(d):
+        # active-lane VALU operations per CU per cycle; GRBM_GUI_ACTIVE is
+        # summed over the 8 XCDs, so one XCD's count is the kernel's cycles
+        if "SQ_THREAD_CYCLES_VALU" in d and d.get("GRBM_GUI_ACTIVE"):
+            return d["SQ_THREAD_CYCLES_VALU"] / CU_NUM / (d["GRBM_GUI_ACTIVE"] / 8)
+        return None
+
+    # the FMA-chain probe (k_fma_probe) in the same run is the peak
+    peak = next((lanes_per_cu_cycle(d) for k, d in merged.items() if k.startswith("k_fma_probe")),
+                None)
+    print("| kernel | dispatches | " + " | ".join(names) +
+          " | VALU lane-ops per CU-cycle | of the FMA-chain peak | lanes active per VALU instruction |")
+    print("|---|---:|" + "---:|" * len(names) + "---:|---:|---:|")
     out = {}
     for k in sorted(merged):
         d = merged[k]
-        busy = util = None
-        if "SQ_ACTIVE_INST_VALU" in d and "GRBM_GUI_ACTIVE" in d:
-            # GRBM_GUI_ACTIVE is summed over the 8 XCDs; VALUBusy divides by one
-            busy = 100.0 * d["SQ_ACTIVE_INST_VALU"] / CU_NUM / (d["GRBM_GUI_ACTIVE"] / 8)
-        if "SQ_THREAD_CYCLES_VALU" in d and d.get("SQ_ACTIVE_INST_VALU"):
-            util = 100.0 * d["SQ_THREAD_CYCLES_VALU"] / (d["SQ_ACTIVE_INST_VALU"] * 64)
-        out[k] = dict(d, valu_busy_pct=busy, valu_util_pct=util)
+        lanes = lanes_per_cu_cycle(d)
+        frac = lanes / peak if lanes is not None and peak else None
+        util = None
+        if "SQ_THREAD_CYCLES_VALU" in d and d.get("SQ_INSTS_VALU"):
+            util = 100.0 * d["SQ_THREAD_CYCLES_VALU"] / (d["SQ_INSTS_VALU"] * 64)
+        out[k] = dict(d, valu_lanes_per_cu_cycle=lanes, valu_frac_of_peak=frac, valu_util_pct=util)
         cells = " | ".join(f"{d.get(c, float('nan')):.4g}" for c in names)
         print(f"| {k} | {int(d['dispatches'])} | {cells} | "
-              f"{'' if busy is None else f'{busy:.1f}%'} | "
+              f"{'' if lanes is None else f'{lanes:.1f}'} | "
+              f"{'' if frac is None else f'{100 * frac:.1f}%'} | "
               f"{'' if util is None else f'{util:.1f}%'} |")
     print("\n```json\n" + json.dumps({"counters": out}, indent=1) + "\n```")
 
