@@ -390,6 +390,23 @@ def test_stage_timings_and_kernel_times(gsph, po):
     s.close()
 
 
+def test_measurement_probes(gsph):
+    """The calibration kernels of the roofline figures: the copy probe moves
+    its bytes at a plausible HBM rate, the FMA-chain probe reports exactly
+    blocks x 256 threads x iterations x 64 lane operations."""
+    banner("probes")
+    ms = gsph.probe_copy(256 * 2 ** 20, 3)
+    gbs = 2 * 256 * 2 ** 20 / (ms * 1e-3) / 1e9
+    assert 500 < gbs < 8000, gbs
+    ms, ops = gsph.probe_valu(200)
+    blocks = ops / (256 * 200 * 64)                      # 8 blocks of 256 per CU
+    assert blocks == int(blocks) and int(blocks) % 8 == 0 and blocks >= 8 and ms > 0
+    tops = ops / (ms * 1e-3) / 1e12
+    assert 5 < tops < 80, tops
+    with pytest.raises(gsph.GsphError):
+        gsph.probe_valu(0)
+
+
 def test_tuned_split_same_physics(gsph, po):
     op, pos = mixed_scene(po)
     n = len(pos)
