@@ -204,12 +204,7 @@ __global__ __launch_bounds__(kNT) void k_movers(ArrangeArgs a) {
     if (wave == 0) {   // s_wcnt is ordered by the barriers of the scan above
         const int w = tile * 64 + lane;
         const uint32_t c = s_wcnt[lane];
-        uint32_t x = c;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
+        const uint32_t x = wave_incl_scan(c);
         s_wpre[lane] = pre + x - c;
         if (w < nwords) a.mpre[w] = pre + x - c;
         if (lane == 63 && tile == (int)gridDim.x - 1) a.dc[kCntMovers] = (int)(pre + x);
@@ -699,12 +694,7 @@ __global__ __launch_bounds__(kNT) void k_cells(ArrangeArgs a) {
     static_assert(kSlots * (kNT / 64) == 64, "one wave scans the slot counts");
     if (wave == 0) {   // (slot, wave) order is particle order: one wave scan
         const uint32_t v = (&s_base[0][0])[lane];
-        uint32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
+        const uint32_t x = wave_incl_scan(v);
         (&s_base[0][0])[lane] = pre + x - v;
     }
     __syncthreads();
@@ -807,12 +797,7 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
         __syncthreads();
         if (tid < 64) {   // prefix of the 32 word populations (first wave)
             const uint32_t pc = tid < kRound / 32 ? (uint32_t)__popc(s_bits[tid]) : 0u;
-            uint32_t x = pc;
-#pragma unroll
-            for (int o = 1; o < 32; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o, 64);
-                if (tid >= o) x += y;
-            }
+            const uint32_t x = wave_incl_scan(pc);
             if (tid < kRound / 32) s_pre[tid] = x - pc;
             if (tid == kRound / 32 - 1) s_pre[kRound / 32] = x;
         }

@@ -84,19 +84,28 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Inclusive scan over the 64 lanes of a wave (all lanes active) with DPP, in
+// registers: Hillis-Steele over row_shr 1, 2, 4, 8 inside each row of 16
+// lanes (lanes whose source falls below the row read 0), then lane 15 of
+// rows 0 and 2 broadcast into rows 1 and 3 (row_bcast:15) and lane 31 into
+// rows 2 and 3 (row_bcast:31); rows outside the row mask add 0.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
 // Block-wide exclusive scan of one uint32 per thread (blockDim.x = NT, a
 // multiple of 64).  Returns the exclusive prefix, writes the block total.
 template <int NT>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_tmp,
                                                     uint32_t &total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    // inclusive wave scan by shuffles
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const uint32_t x = wave_incl_scan(v);
     if (lane == 63) s_tmp[wave] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
