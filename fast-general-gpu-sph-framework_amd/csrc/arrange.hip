@@ -741,7 +741,7 @@ __device__ __forceinline__ int layer_bound(const uint32_t *hash, int lo, int hi,
 
 // Dense begin[numc+1] (a cell's count is begin[c+1] - begin[c]; no dense
 // count array is written).
-// Block q fills cells [4096 q, 4096 q + 4096) in 4 rounds of 1024; each
+// Block q fills cells [2048 q, 2048 q + 2048) in 2 rounds of 1024; each
 // thread owns 4 consecutive cells (16-byte stores).  The occupied cells of a
 // round are marked in a 1024-bit LDS map; a thread's cursor into the
 // occupied-cell list is the map's population below its cells.

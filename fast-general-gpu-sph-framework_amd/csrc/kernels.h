@@ -17,7 +17,7 @@ struct f3 {
 
 constexpr int kSortTile = 4096;     // keys per radix tile (256 threads x 16)
 constexpr int kCellTile = 4096;     // sorted particles per cell-table tile
-constexpr int kFillChunk = 4096;    // cells per fill block
+constexpr int kFillChunk = 2048;    // cells per fill block
 constexpr int kSplitTile = 1024;    // occupied cells per split tile
 constexpr int kMaskWords = 19;       // accepted-pair mask + row plan words per TRA particle
 
