@@ -988,6 +988,8 @@ int gsph_add_particles(gsph_ctx *ctx, int count, const float *pos4, const float 
     constexpr long long kMaxParticles = 0x7fffff00LL;
     const long long n_new = (long long)ctx->n + count;
     if (n_new > kMaxParticles) return fail(ctx, GSPH_ERR_CAPACITY, "too many particles");
+    if (ctx->worlds.empty() || ctx->scenes.empty())
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "context has no scene box");
     const auto &w = ctx->worlds[0];
     std::vector<float> p4((size_t)count * 4);
     for (int k = 0; k < count; ++k) {
