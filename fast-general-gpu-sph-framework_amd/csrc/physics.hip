@@ -612,8 +612,11 @@ __device__ __forceinline__ void force_pair(const float4 &pi, const float4 &ei,
     const float dist = d2 * inv;
     const float V = pj.w;
     const float t = h2_minus_r2(rx, ry, rz, h2d);
-    // h - r without cancellation; the reciprocal is within 1 ulp
-    const float kr = t * __builtin_amdgcn_rcpf(c.kernel_h + dist);
+    // h - r without cancellation: one Newton step on kr (h + r) = t from
+    // h - r, with 1/(2h) for 1/(h + r); it leaves dr kr / h of the error dr of
+    // r, about 2 ulp of kr over the whole range
+    const float kr0 = c.kernel_h - dist;
+    const float kr = fmaf(fmaf(-kr0, c.kernel_h + dist, t), 0.5f / c.kernel_h, kr0);
     const float Vkr = V * kr;
     const float pw = Vkr * (ei.w + ej.w) * kr * inv;
     A.px -= rx * pw;
