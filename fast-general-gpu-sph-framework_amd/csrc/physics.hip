@@ -24,6 +24,13 @@
 
 #include <algorithm>
 
+// No implicit fusion of a * b + c anywhere in this file: the pair bodies are
+// inlined into several paths (TRA rows, SMS batches, pool rounds, fused and
+// separate integration), and each copy must round exactly like the others
+// whatever the compiler does with the code around it.  The fused
+// multiply-adds the kernels want are written as fmaf.
+#pragma clang fp contract(off)
+
 namespace gsph {
 namespace {
 
@@ -607,30 +614,29 @@ __device__ __forceinline__ void force_pair(const float4 &pi, const float4 &ei,
                                            const gsph_constants &c, double h2d,
                                            ForceAcc &A) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-    const float d2 = rx * rx + ry * ry + rz * rz;
+    const float d2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
     const float inv = __frsqrt_rn(d2);
-    const float dist = d2 * inv;
     const float V = pj.w;
     const float t = h2_minus_r2(rx, ry, rz, h2d);
     // h - r without cancellation: one Newton step on kr (h + r) = t from
     // h - r, with 1/(2h) for 1/(h + r); it leaves dr kr / h of the error dr of
-    // r, about 2 ulp of kr over the whole range
-    const float kr0 = c.kernel_h - dist;
-    const float kr = fmaf(fmaf(-kr0, c.kernel_h + dist, t), 0.5f / c.kernel_h, kr0);
+    // r, about 2 ulp of kr over the whole range (r = d2 / sqrt(d2))
+    const float kr0 = fmaf(-d2, inv, c.kernel_h);
+    const float kr = fmaf(fmaf(-kr0, fmaf(d2, inv, c.kernel_h), t), 0.5f / c.kernel_h, kr0);
     const float Vkr = V * kr;
     const float pw = Vkr * (ei.w + ej.w) * kr * inv;
-    A.px -= rx * pw;
-    A.py -= ry * pw;
-    A.pz -= rz * pw;
-    A.vx += (ej.x - ei.x) * Vkr;
-    A.vy += (ej.y - ei.y) * Vkr;
-    A.vz += (ej.z - ei.z) * Vkr;
+    A.px = fmaf(-rx, pw, A.px);
+    A.py = fmaf(-ry, pw, A.py);
+    A.pz = fmaf(-rz, pw, A.pz);
+    A.vx = fmaf(ej.x - ei.x, Vkr, A.vx);
+    A.vy = fmaf(ej.y - ei.y, Vkr, A.vy);
+    A.vz = fmaf(ej.z - ei.z, Vkr, A.vz);
     const float Vt = V * t;
     const float gw = Vt * t;
-    A.gx += rx * gw;
-    A.gy += ry * gw;
-    A.gz += rz * gw;
-    A.l += Vt * (d2 - 0.75f * t);
+    A.gx = fmaf(rx, gw, A.gx);
+    A.gy = fmaf(ry, gw, A.gy);
+    A.gz = fmaf(rz, gw, A.gz);
+    A.l = fmaf(Vt, fmaf(-0.75f, t, d2), A.l);
 }
 
 // One particle's integration (knIntegrateVelocityE, integration_kernels.cu:
