@@ -970,6 +970,17 @@ int grow_particles(gsph_ctx *ctx, int cap) {
         return rc;
     }
     for (void *p : saved.allocs) (void)hipFree(p);
+    // a captured step refers to the old buffers: never replay it
+    if (ctx->graph_exec) {
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
+    if (ctx->graph) {
+        (void)hipGraphDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    ctx->grid_valid = false;
+    ctx->masks_valid = false;
     return GSPH_OK;
 }
 
