@@ -1486,7 +1486,7 @@ int gsph_probe_copy(double bytes, int reps, float *ms) {
 }
 
 int gsph_probe_valu(int iters, float *ms, double *lane_ops) {
-    if (iters <= 0 || iters > (1 << 24)) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe length");
+    if (iters <= 0 || iters > (1 << 20)) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe length");
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)

@@ -334,7 +334,7 @@ int gsph_probe_copy(double bytes, int reps, float *ms);
  * FP32 FMA chains per lane on 8 waves per SIMD, iters x 64 FMAs per lane.
  * Returns the launch's ms and its lane operations (threads x FMAs): the peak
  * VALU rate against which the density and force kernels' active-lane
- * operations (SQ_THREAD_CYCLES_VALU) are measured. */
+ * operations (SQ_THREAD_CYCLES_VALU) are measured.  iters <= 2^20 (about 1 s). */
 int gsph_probe_valu(int iters, float *ms, double *lane_ops);
 /* Test hook: the next call that launches kernels fails with GSPH_ERR_INJECTED
  * after launching nothing (failure-path tests of the serving loop). */
