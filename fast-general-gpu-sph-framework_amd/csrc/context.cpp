@@ -1456,7 +1456,7 @@ int gsph_kernel_times(gsph_ctx *ctx, char *names, int names_cap, float *ms, int 
 }
 
 int gsph_probe_copy(double bytes, int reps, float *ms) {
-    if (bytes <= 0 || reps <= 0) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe size");
+    if (bytes <= 0 || reps <= 0 || reps > 1000) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe size");
     const size_t n = (size_t)(bytes / 16);
     float4 *a = nullptr, *b = nullptr;
     hipStream_t s = nullptr;
