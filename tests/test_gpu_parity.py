@@ -403,8 +403,9 @@ def test_measurement_probes(gsph):
     assert blocks == int(blocks) and int(blocks) % 8 == 0 and blocks >= 8 and ms > 0
     tops = ops / (ms * 1e-3) / 1e12
     assert 5 < tops < 80, tops
-    with pytest.raises(gsph.GsphError):
-        gsph.probe_valu(0)
+    for bad in (0, (1 << 20) + 1):
+        with pytest.raises(gsph.GsphError):
+            gsph.probe_valu(bad)
 
 
 def test_tuned_split_same_physics(gsph, po):
