@@ -7,7 +7,9 @@
 
 `stats` turns a `--kernel-trace --stats` table into per-kernel average
 durations; `counters` averages every counter per dispatch of each kernel and
-adds VALU busy / VALU utilisation when their inputs were collected; `traffic`
+adds the active-lane VALU operations per CU-cycle, their share of the FMA-chain
+probe's rate (`k_fma_probe`, run by bench.py in the same profile) and the
+lanes active per VALU instruction, when their inputs were collected; `traffic`
 converts FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into HBM bytes per launch,
 each scaled by what the same counter reads for the plain copy kernel `k_copy`
 over a buffer of known size (bench.py --probe-copy B).  Output is Markdown
