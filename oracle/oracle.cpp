@@ -351,12 +351,15 @@ int orc_arrange(const orc_params *p, int n, const float *pos4, int *hash,
     for (int f = 0; f < T; f += 2) {
         const int s = f + 1;
         if (s >= T) {
+            // judgeTask's odd tail (:976-991) sets is_same, cellid,
+            // cell_pos, cell_begin, the count and p_offset + 32 of the pad and
+            // leaves its other bytes as they were in device memory (stale).
+            // Those bytes are defined here as padIndependentSmsTasks
+            // (:1024-1032, the build without the historical judgeTask)
+            // defines them: a copy of the first task.  The pair statistics
+            // read the pad's xxx (compact vs widened).
             tk[f].is_same = 1;
-            tk[s].is_same = 1;
-            tk[s].cellid = tk[f].cellid;
-            std::memcpy(tk[s].cell_pos, tk[f].cell_pos, sizeof(tk[f].cell_pos));
-            tk[s].cell_begin = tk[f].cell_begin;
-            tk[s].cell_particle_count = tk[f].cell_particle_count;
+            tk[s] = tk[f];
             tk[s].p_offset = (uint16_t)(tk[f].p_offset + kSmsTaskParticles);
             continue;
         }

@@ -63,6 +63,7 @@ _SIGS = {
     "orc_integrate_f64": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     "orc_step_f32": (_I, [C.POINTER(OrcParams), _I, _P, _P, _P, _P, _I, _I]),
     "orc_checksum": (_I, [_I, _P, _P]),
+    "orc_pair_stats_of": (_I, [_P, _I, _P, _P]),
 }
 
 _lib = None
@@ -163,6 +164,22 @@ def arrange(p: OrcParams, pos4):
     out["tasks_count"] = T.value
     out["tasks"] = tasks[:T.value + (T.value & 1)]
     return out
+
+
+PAIR_STATS_FIELDS = ("total_pairs", "same_cell_pairs", "is_same_pairs", "full_is_same_pairs",
+                     "partial_is_same_pairs", "compact_is_same_pairs", "widened_is_same_pairs")
+
+
+def pair_stats(tasks, num_tasks, count):
+    """Arrangement::getSmsTaskPairStats over a task array (T tasks plus the
+    odd-tail pad) and the per-cell counts; a dict of the seven fields."""
+    tasks = np.ascontiguousarray(tasks, TASK_DTYPE)
+    if num_tasks > 0 and len(tasks) < num_tasks + (num_tasks & 1):
+        raise ValueError("task array shorter than T plus the pad")
+    count = np.ascontiguousarray(count, np.int32)
+    out = (C.c_int * 7)()
+    load().orc_pair_stats_of(_p(tasks), int(num_tasks), _p(count), C.cast(out, C.c_void_p))
+    return dict(zip(PAIR_STATS_FIELDS, list(out)))
 
 
 def density_f64(p, pos4):

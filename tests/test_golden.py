@@ -71,9 +71,13 @@ def test_gpu_matches_fixture(gsph, po, name):
     for k in ("hash", "perm", "begin", "count", "req", "compact"):
         assert np.array_equal(g[k], f[k]), k
     assert g["middle_raw"] == f["middle_raw"] and g["tasks"].shape[0] == f["tasks"].shape[0]
-    T = int(f["tasks_count"])
+    # every task descriptor byte for byte, the odd-tail pad included
     gt = g["tasks"].view(np.uint8).reshape(-1, 32)
-    assert np.array_equal(gt[:T], f["tasks"][:T])
+    assert np.array_equal(gt, f["tasks"])
+    # the seven pair statistics against the oracle's over the fixture's tasks
+    T = int(f["tasks_count"])
+    exp = po.pair_stats(f["tasks"].reshape(-1).view(po.TASK_DTYPE), T, f["count"])
+    assert s.pair_stats() == exp
     s.density()
     s.force()
     s.integrate()

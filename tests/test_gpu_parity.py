@@ -22,7 +22,6 @@ pytestmark = pytest.mark.gpu
 CHECKSUM_TOL = 2e-6
 TASK_FIELDS = ("is_same", "paired_bounds", "cellid", "cell_pos", "p_offset", "xxi", "xxx",
                "yyi", "yyy", "zzi", "zzz", "cell_begin", "cell_particle_count")
-PAD_FIELDS = ("is_same", "cellid", "cell_pos", "p_offset", "cell_begin", "cell_particle_count")
 
 
 def banner(msg):
@@ -77,13 +76,24 @@ def compare_grid(g, ref):
     assert g["tasks"] is not None and len(g["tasks"]) == len(ref["tasks"])
     T = ref["tasks_count"]
     assert g["tasks"].shape[0] == T + (T & 1)
+    # every field of every descriptor, the odd-tail pad included
     for f in TASK_FIELDS:
-        assert np.array_equal(g["tasks"][f][:T], ref["tasks"][f][:T]), f
-    if T & 1:
-        for f in PAD_FIELDS:
-            assert np.array_equal(g["tasks"][f][T], ref["tasks"][f][T]), "pad " + f
+        assert np.array_equal(g["tasks"][f], ref["tasks"][f]), f
     tra = ref["middle_raw"] if ref["middle_raw"] >= 0 else len(ref["perm"])
     assert g["tra"] == tra
+
+
+def compare_pair_stats(s, ref):
+    """All seven getSmsTaskPairStats fields against the oracle's."""
+    exp = po_pair_stats(ref)
+    got = s.pair_stats()
+    assert got == exp, (got, exp)
+    return got
+
+
+def po_pair_stats(ref):
+    import pyoracle
+    return pyoracle.pair_stats(ref["tasks"], ref["tasks_count"], ref["count"])
 
 
 @pytest.mark.parametrize("name", SCENES)
@@ -97,10 +107,16 @@ def test_arrange_bit_exact(gsph, po, name):
     compare_grid(g, ref)
     if name in ("dense", "dense64", "mixed", "odd_tasks"):
         assert ref["tasks_count"] > 0
+    st = compare_pair_stats(s, ref)
+    print(st, flush=True)
     if name == "dense64":
-        assert s.pair_stats()["same_cell_pairs"] == ref["tasks_count"] // 2
+        assert st["same_cell_pairs"] == ref["tasks_count"] // 2
+        assert st["full_is_same_pairs"] == st["is_same_pairs"] > 0
     if name == "odd_tasks":
         assert ref["tasks_count"] & 1
+        assert st["partial_is_same_pairs"] > 0
+    if name in ("dense", "mixed"):
+        assert st["widened_is_same_pairs"] > 0 and st["compact_is_same_pairs"] > 0
     s.close()
 
 
@@ -117,7 +133,8 @@ def test_arrange_reference_default_scene(gsph, po):
     import pyoracle
     ref = po.arrange(pyoracle.params_from(p), pos)
     compare_grid(g, ref)
-    stats = s.pair_stats()
+    stats = compare_pair_stats(s, ref)
+    print(stats, flush=True)
     T = ref["tasks_count"]
     assert stats["total_pairs"] == (T + 1) // 2 and T > 100000
     s.close()
@@ -139,6 +156,57 @@ def test_one_step_physics(gsph, po, name):
     for k, v in errs.items():
         assert v <= REL_TOL, f"{k} error {v:.3e} > {REL_TOL}"
     s.close()
+
+
+def test_wall_and_speed_clamps_one_step(gsph, po):
+    """Particles that cross a wall in one step (put back at bound -+ (h +
+    0.01), velocity reflected by -0.5) and whose speed is clamped to 6 m/s
+    (integration_kernels.cu:300-326), next to a resting block: one GPU step
+    within the bar of the FP64 oracle, and the clamps' known outcomes."""
+    op = make_params(h=0.02, world=(0.64, 0.64, 0.64))
+    c = po.derive(op)
+    kern = float(np.float32(float(np.float32(0.02)) + 0.01))
+    blk = po.scene_block([0.26, 0.26, 0.26], 8, 8, 8, 0.01, 0.01, 11)
+    fast, vfast, expect = [], [], []
+    k = 0
+    for axis in range(3):
+        for side in (1, -1):
+            bound = float(c.bound_max[axis]) if side > 0 else float(c.bound_min[axis])
+            q = [0.1 + 0.07 * k, 0.55 - 0.07 * k, 0.12 + 0.065 * k]
+            q[axis] = bound - side * 1e-4
+            v = [0.0, 0.0, 0.0]
+            v[axis] = 10.0 * side
+            fast.append(q + [0.0])
+            vfast.append(v)
+            expect.append((axis, side, bound))
+            k += 1
+    # interior particles faster than 6 m/s (speed clamp only)
+    fast += [[0.1, 0.5, 0.5, 0.0], [0.5, 0.1, 0.12, 0.0]]
+    vfast += [[3.0, -4.0, 5.0], [0.0, 9.0, -2.0]]
+    pos = np.concatenate([blk, np.array(fast, np.float32)])
+    vel = np.concatenate([np.zeros((len(blk), 3), np.float32), np.array(vfast, np.float32)])
+    n = len(pos)
+    banner(f"clamps n={n}")
+    s = gsph.System(to_gsph_params(gsph, op), pos, vel)
+    s.step(1)
+    pos1, vel1, ev1, acc1 = s.read_state()
+    s.close()
+    ref = po.fp64_step(op, pos, vel, np.zeros((n, 4), np.float32))
+    errs = step_errors(pos1, vel1, ev1, acc1, ref, op)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for key, v in errs.items():
+        assert v <= REL_TOL, f"{key} error {v:.3e} > {REL_TOL}"
+    # the sorted slot of each fast particle: its index in the oracle's order
+    slot = np.argsort(ref["grid"]["perm"])
+    for j, (axis, side, bound) in enumerate(expect):
+        i = slot[len(blk) + j]
+        assert pos1[i, axis] == np.float32(np.float32(bound) - side * np.float32(kern))
+        assert vel1[i, axis] * side < 0
+        assert abs(4 * vel1[i, axis] ** 2 + sum(vel1[i, a] ** 2 for a in range(3) if a != axis)
+                   - 36.0) < 1e-4
+    for j in range(2):
+        i = slot[len(blk) + len(expect) + j]
+        assert abs(np.linalg.norm(vel1[i].astype(np.float64)) - 6.0) < 1e-5
 
 
 def _dump(tag, **arrays):
@@ -591,6 +659,8 @@ def test_config2_full_size(gsph, po):
     g = s.read_grid()
     ref = po.arrange(op, pos)
     compare_grid(g, ref)
+    st = compare_pair_stats(s, ref)
+    print(st, flush=True)
     assert ref["tasks_count"] > 100000 and 0 < ref["middle_raw"] < n
     s.density()
     s.force()

@@ -379,6 +379,94 @@ def test_integrate_free_fall_and_walls(po):
     assert vo[1, 1] == pytest.approx(-1.0 + ay * dt, rel=1e-9)
 
 
+def test_integrate_speed_clamp_closed_form(po):
+    """|v|^2 > 36 after the Euler update: the speed is scaled to exactly 6
+    with the direction kept (integration_kernels.cu:300-306).  Far from every
+    wall, no force: v' = v + g dt, then v' * 6 / |v'|."""
+    p = make_params(h=0.02, world=(1.0, 1.0, 1.0), dt=1e-4)
+    pos = np.array([[0.5, 0.5, 0.5, 0.001], [0.4, 0.6, 0.5, 0.001]], np.float32)
+    vel = np.array([[0.0, 0.0, 10.0], [3.0, -4.0, 5.0]], np.float32)   # |v|^2 = 100, 50
+    ev = np.zeros((2, 4), np.float32)
+    po_, vo, eo = po.integrate_f64(p, pos, vel, ev, np.zeros((2, 3)), np.array([1000.0, 1000.0]))
+    g = float(np.float32(-9.81))
+    dt = float(np.float32(1e-4))
+    for i in range(2):
+        v1 = vel[i].astype(np.float64) + np.array([0.0, g * dt, 0.0])
+        assert np.linalg.norm(vo[i]) == pytest.approx(6.0, rel=1e-14)
+        np.testing.assert_allclose(vo[i] / 6.0, v1 / np.linalg.norm(v1), rtol=1e-14, atol=1e-16)
+        np.testing.assert_allclose(po_[i, :3], pos[i, :3].astype(np.float64) + vo[i] * dt,
+                                   rtol=1e-15)
+        np.testing.assert_allclose(eo[i, :3], vo[i] / 2, rtol=1e-15)
+
+
+@pytest.mark.parametrize("axis,side", [(0, 1), (1, -1), (2, 1), (2, -1)])
+def test_integrate_wall_clamp_closed_form(po, axis, side):
+    """A particle that crosses a wall in one step is put back at
+    bound -+ (h + 0.01) and its velocity component reflected by
+    wall_damping = -0.5 (integration_kernels.cu:308-326).  Heading into the
+    wall at 10 m/s from 1e-4 m inside it, the speed clamp leaves |v| = 6
+    before the wall, so 4 v_axis'^2 + (other components)^2 = 36 after it."""
+    h = 0.02
+    p = make_params(h=h, world=(1.0, 1.0, 1.0), dt=1e-4)
+    c = po.derive(p)
+    kern = float(np.float32(float(np.float32(h)) + 0.01))
+    bound = float(c.bound_max[axis]) if side > 0 else float(c.bound_min[axis])
+    pos = np.array([[0.5, 0.5, 0.5, 0.0]], np.float32)
+    pos[0, axis] = np.float32(bound - side * 1e-4)
+    vel = np.zeros((1, 3), np.float32)
+    vel[0, axis] = 10.0 * side
+    ev = np.zeros((1, 4), np.float32)
+    po_, vo, eo = po.integrate_f64(p, pos, vel, ev, np.zeros((1, 3)), np.array([1000.0]))
+    assert po_[0, axis] == bound - side * kern
+    assert vo[0, axis] * side < 0                                 # reflected
+    others = [a for a in range(3) if a != axis]
+    assert 4 * vo[0, axis] ** 2 + sum(vo[0, a] ** 2 for a in others) == pytest.approx(36.0, rel=1e-13)
+    assert abs(vo[0, axis]) == pytest.approx(3.0, rel=1e-6)
+
+
+def test_pair_stats_known_answers(po):
+    """getSmsTaskPairStats (sph_arrangement.cu:1595-1636) on hand-counted
+    task lists.  The 70-particle cell of test_arrange_tasks_known_answer has
+    3 tasks + pad: pair (0,1) is one cell, second task holds 70 - 32 = 38 >= 32
+    particles (full), first.xxi 0 != second.xxx 2 (widened); the odd-tail
+    pair (2, pad) is one cell, the pad holds 70 - 96 < 32 (partial) and
+    copies task 2's x bounds 2..2 (compact)."""
+    p = make_params(h=0.04, world=(0.12, 0.12, 0.12))
+    rows = []
+    for (mx, my, mz), k in [((0, 0, 0), 20), ((0, 2, 1), 20), ((2, 3, 3), 30)]:
+        for j in range(k):
+            rows.append([0.04 + 0.01 * mx + 0.0005 + 1e-5 * j, 0.04 + 0.01 * my + 0.005,
+                         0.04 + 0.01 * mz + 0.005, 0])
+    pos = np.array(rows, np.float32)[::-1].copy()
+    g = po.arrange(p, pos)
+    st = po.pair_stats(g["tasks"], g["tasks_count"], g["count"])
+    assert st == dict(total_pairs=2, same_cell_pairs=2, is_same_pairs=2, full_is_same_pairs=1,
+                      partial_is_same_pairs=1, compact_is_same_pairs=1, widened_is_same_pairs=1)
+    # 64 particles in micro x layer 1: two tasks, 64 - 32 = 32 (full), x bounds
+    # 1..1 both (compact)
+    rows = [[0.04 + 0.015, 0.04 + 0.005 + 0.01 * (j % 4), 0.04 + 0.01 * (j // 16) + 0.005, 0]
+            for j in range(64)]
+    g = po.arrange(p, np.array(rows, np.float32))
+    st = po.pair_stats(g["tasks"], g["tasks_count"], g["count"])
+    assert st == dict(total_pairs=1, same_cell_pairs=1, is_same_pairs=1, full_is_same_pairs=1,
+                      partial_is_same_pairs=0, compact_is_same_pairs=1, widened_is_same_pairs=0)
+    # two cells of 30 particles each ((30 + 27) >> 5 = 1 task each, no
+    # neighbours): one pair of two cells, not isSame
+    p3 = make_params(h=0.04, world=(0.24, 0.12, 0.12))
+    rng = np.random.default_rng(3)
+    a = np.zeros((60, 4), np.float32)
+    a[:30, :3] = rng.uniform([0.042, 0.042, 0.042], [0.078, 0.078, 0.078], (30, 3))
+    a[30:, :3] = rng.uniform([0.162, 0.042, 0.042], [0.198, 0.078, 0.078], (30, 3))
+    g = po.arrange(p3, a)
+    assert g["tasks_count"] == 2
+    st = po.pair_stats(g["tasks"], g["tasks_count"], g["count"])
+    assert st == dict(total_pairs=1, same_cell_pairs=0, is_same_pairs=0, full_is_same_pairs=0,
+                      partial_is_same_pairs=0, compact_is_same_pairs=0, widened_is_same_pairs=0)
+    # no tasks: all zero
+    st = po.pair_stats(g["tasks"][:0], 0, g["count"])
+    assert all(v == 0 for v in st.values())
+
+
 def test_checksum_definition(po):
     pos = np.arange(2048 * 4, dtype=np.float32).reshape(2048, 4) * np.float32(1e-3)
     out = po.checksum(pos)
