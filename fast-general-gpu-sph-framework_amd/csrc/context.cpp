@@ -279,6 +279,7 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.sms_pool_rows = ctx->sms_pool_rows;
     p.sms_off = ctx->sms_off;
     p.dcw = ctx->dc;
+    p.wall_margin = (float)((double)ctx->c.kernel_h + 0.01);
     p.fuse_integrate = 0;
     p.next_keys = ctx->mkey[1];
     return p;
@@ -366,27 +367,31 @@ int invalidate_order(gsph_ctx *ctx) {
     return GSPH_OK;
 }
 
+// Reports one raised device error flag and clears only that flag, so that a
+// second condition raised in the same step (say a capacity overflow next to a
+// particle that left its box) is reported by the next call instead of lost.
 int collect_errors(gsph_ctx *ctx) {
     int flags = 0;
     HIP_TRY(ctx, hipMemcpyAsync(&flags, ctx->dc + kCntErr, sizeof(int),
                                 hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (flags) {
-        const int zero = 0;
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->dc + kCntErr, &zero, sizeof(int),
-                                    hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (flags & kErrOutOfDomain)
-            return fail(ctx, GSPH_ERR_OUT_OF_DOMAIN,
-                        "a particle left the grid (or is not finite)");
-        if (flags & kErrCapacity)
-            return fail(ctx, GSPH_ERR_CAPACITY,
-                        "task capacity exceeded or a cell holds more than 65535 particles");
-        if (flags & kErrSortStall)
-            return fail(ctx, GSPH_ERR_SPIN_TIMEOUT,
-                        "a sort tile waited too long for its predecessor");
-    }
-    return GSPH_OK;
+    if (!flags) return GSPH_OK;
+    const int bit = (flags & kErrOutOfDomain) ? kErrOutOfDomain
+                    : (flags & kErrCapacity)  ? kErrCapacity
+                    : (flags & kErrSortStall) ? kErrSortStall
+                                              : flags;
+    const int rest = flags & ~bit;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->dc + kCntErr, &rest, sizeof(int),
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (bit == kErrOutOfDomain)
+        return fail(ctx, GSPH_ERR_OUT_OF_DOMAIN, "a particle left the grid (or is not finite)");
+    if (bit == kErrCapacity)
+        return fail(ctx, GSPH_ERR_CAPACITY,
+                    "task capacity exceeded or a cell holds more than 65535 particles");
+    if (bit == kErrSortStall)
+        return fail(ctx, GSPH_ERR_SPIN_TIMEOUT, "a sort tile waited too long for its predecessor");
+    return fail(ctx, GSPH_ERR_HIP, "unknown device error flag " + std::to_string(bit));
 }
 
 int bits_for(long long maxval) {
@@ -1453,70 +1458,6 @@ int gsph_kernel_times(gsph_ctx *ctx, char *names, int names_cap, float *ms, int 
         names[names_cap - 1] = 0;
     }
     return GSPH_OK;
-}
-
-int gsph_probe_copy(double bytes, int reps, float *ms) {
-    if (bytes <= 0 || reps <= 0 || reps > 1000) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe size");
-    const size_t n = (size_t)(bytes / 16);
-    float4 *a = nullptr, *b = nullptr;
-    hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = GSPH_OK;
-    if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess ||
-        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        rc = fail(nullptr, GSPH_ERR_HIP, "probe allocation failed");
-    } else {
-        (void)hipMemsetAsync(a, 0, n * 16, s);
-        launch_copy(a, b, n, s);
-        (void)hipEventRecord(e0, s);
-        for (int r = 0; r < reps; ++r) launch_copy(a, b, n, s);
-        (void)hipEventRecord(e1, s);
-        if (hipStreamSynchronize(s) != hipSuccess) rc = fail(nullptr, GSPH_ERR_HIP, "probe failed");
-        float t = 0.f;
-        (void)hipEventElapsedTime(&t, e0, e1);
-        if (ms) *ms = t / reps;
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (s) (void)hipStreamDestroy(s);
-    if (a) (void)hipFree(a);
-    if (b) (void)hipFree(b);
-    return rc;
-}
-
-int gsph_probe_valu(int iters, float *ms, double *lane_ops) {
-    if (iters <= 0 || iters > (1 << 20)) return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad probe length");
-    int dev = 0, cus = 0;
-    (void)hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        return fail(nullptr, GSPH_ERR_NO_DEVICE, "no device");
-    const int blocks = 8 * cus;   // 8 blocks of 4 waves per CU: 8 waves per SIMD
-    float *out = nullptr;
-    hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = GSPH_OK;
-    if (hipMalloc(&out, sizeof(float) * (size_t)blocks * kProbeThreads) != hipSuccess ||
-        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        rc = fail(nullptr, GSPH_ERR_HIP, "probe allocation failed");
-    } else {
-        launch_fma_probe(out, blocks, 16, s);
-        (void)hipEventRecord(e0, s);
-        launch_fma_probe(out, blocks, iters, s);
-        (void)hipEventRecord(e1, s);
-        if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
-            rc = fail(nullptr, GSPH_ERR_HIP, "probe failed");
-        float t = 0.f;
-        (void)hipEventElapsedTime(&t, e0, e1);
-        if (ms) *ms = t;
-        if (lane_ops) *lane_ops = (double)blocks * kProbeThreads * (double)iters * 64.0;
-    }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (s) (void)hipStreamDestroy(s);
-    if (out) (void)hipFree(out);
-    return rc;
 }
 
 int gsph_inject_fault(gsph_ctx *ctx, int after_calls) {

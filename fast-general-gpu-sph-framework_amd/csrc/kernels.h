@@ -111,6 +111,9 @@ struct PhysicsArgs {
     int sms_pool_rows;
     int *sms_off;          // [task_cap / 2 + 1]
     int *dcw;              // writable alias of dc
+    float wall_margin;     // the integrator's wall margin h + 0.01: the reference adds
+                           // the double literal to the float h and stores the sum as a
+                           // float (integration_kernels.cu:256)
     int fuse_integrate;    // force also integrates (the full step); else k_integrate
     uint32_t *next_keys;   // [n] with fuse_integrate: micro id of each integrated particle
 };
@@ -137,9 +140,5 @@ void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_checksum(const float4 *pos, int n, double *partial, int nblocks,
                      double *out5, hipStream_t s);
 void launch_init_state(float4 *ev, f3 *acc, int n, hipStream_t s);
-void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s);
-constexpr int kProbeThreads = 256;
-// out: blocks * kProbeThreads floats
-void launch_fma_probe(float *out, int blocks, int iters, hipStream_t s);
 
 }  // namespace gsph

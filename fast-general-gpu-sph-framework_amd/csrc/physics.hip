@@ -100,6 +100,15 @@ __device__ __forceinline__ void plan_rows(const SceneGeom &g, const gsph_constan
     const int cy = my >> 2, cz = mz >> 2;
     const float h = c.cell_size;
     const float rat = 4.f * c.inv_cell_size;
+    // Rounding slack of the slab distances and the reach.  The hash puts a
+    // cell boundary where fl(y * rat) crosses a multiple of 4, which differs
+    // from fl(cy * h) by a few ulp of y; fl(x -+ reach) rounds by half an ulp
+    // of x.  2^-20 of the coordinate (8 ulp) plus the fixed margins below
+    // (1e-3 relative on h^2, 1e-3 on the reach, 1e-4 h) keep the pruned set a
+    // superset of the pairs the FP32 test accepts for every grid up to 16383
+    // cells per axis.
+    const float sy = (fabsf(y) + h) * 0x1p-20f, sz = (fabsf(z) + h) * 0x1p-20f;
+    const float sx = (fabsf(x) + h) * 0x1p-20f;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const int dy = r % 3 - 1, dz = r / 3 - 1;
@@ -109,16 +118,14 @@ __device__ __forceinline__ void plan_rows(const SceneGeom &g, const gsph_constan
         rp.live[r] = ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz;
         int gl = mx - 4, gr = mx + 4;
         if (prune && rp.live[r]) {
-            const float ey = dy < 0 ? y - (float)cy * h : dy > 0 ? (float)(cy + 1) * h - y : 0.f;
-            const float ez = dz < 0 ? z - (float)cz * h : dz > 0 ? (float)(cz + 1) * h - z : 0.f;
+            const float ey = dy < 0 ? y - (float)cy * h - sy : dy > 0 ? (float)(cy + 1) * h - y - sy : 0.f;
+            const float ez = dz < 0 ? z - (float)cz * h - sz : dz > 0 ? (float)(cz + 1) * h - z - sz : 0.f;
             const float d2 = fmaxf(ey, 0.f) * fmaxf(ey, 0.f) + fmaxf(ez, 0.f) * fmaxf(ez, 0.f);
-            // margins: 1e-3 relative on h^2 and the reach plus 1e-4 h absolute,
-            // far above the FP32 rounding of coordinates up to 2^14 h
             const float lim = c.kernel_2 * 1.001f;
             if (d2 > lim) {
                 rp.live[r] = false;
             } else {
-                const float reach = sqrtf(fmaxf(lim - d2, 0.f)) * 1.001f + 1e-4f * h;
+                const float reach = sqrtf(fmaxf(lim - d2, 0.f)) * 1.001f + 1e-4f * h + sx;
                 gl = max(gl, (int)floorf((x - reach) * rat));
                 gr = min(gr, (int)floorf((x + reach) * rat));
             }
@@ -596,28 +603,49 @@ struct ForceAcc {
     float px, py, pz, vx, vy, vz, gx, gy, gz, l;
 };
 
-// h^2 - r^2 rounded once.  Near r = h the plain FP32 difference cancels (its
-// error is ~1 ulp of h^2, i.e. relative h^2/(h^2 - r^2)), which makes the
-// kr^2 and (h^2 - r^2)^2 terms of a pair near the kernel edge inaccurate.
-// The coordinate differences of neighbours are exact (Sterbenz), their
-// squares are exact in FP64 and their sum is within 2^-52 r^2; h^2 is exact
-// in FP64 for an FP32 h, so the difference is correct to FP32 rounding.
-__device__ __forceinline__ float h2_minus_r2(float rx, float ry, float rz, double h2d) {
-    const double x = rx, y = ry, z = rz;
-    return (float)(h2d - fma(x, x, fma(y, y, z * z)));
+// h^2 - r^2 in FP32, correct to about one rounding of the result.  Near
+// r = h the plain difference h^2 - fl(r^2) cancels: its error is ~1 ulp of h^2,
+// i.e. relative h^2/(h^2 - r^2), which makes the kr^2 and (h^2 - r^2)^2 terms of
+// a pair near the kernel edge inaccurate.  Here every rounding is recovered:
+// the coordinate differences of neighbours are exact (Sterbenz); each square
+// is split exactly into a + ea by one FMA; the two sums of the large parts are
+// error-free transformations (Fast2Sum after ordering by max/min, all terms
+// non-negative); h^2 is the exact pair hh.hi + hh.lo; hh.hi - S is exact by
+// Sterbenz whenever S lies in [h^2/2, 2h^2] (and where it does not, t is large
+// and its rounding harmless); the small parts are added last.  S, the rounded
+// r^2, is returned for the reciprocal square root.
+struct HH {
+    float hi, lo;   // h^2 = hi + lo exactly
+};
+
+__device__ __forceinline__ HH exact_h2(float h) {
+    const float hi = h * h;
+    return {hi, fmaf(h, h, -hi)};
 }
 
+__device__ __forceinline__ float h2_minus_r2(float rx, float ry, float rz, HH hh, float &S) {
+    const float a = rx * rx, ea = fmaf(rx, rx, -a);
+    const float b = ry * ry, eb = fmaf(ry, ry, -b);
+    const float c = rz * rz, ec = fmaf(rz, rz, -c);
+    const float p = fmaxf(a, b), q = fminf(a, b);
+    const float s = p + q, es = q - (s - p);
+    const float p2 = fmaxf(s, c), q2 = fminf(s, c);
+    S = p2 + q2;
+    const float eS = q2 - (S - p2);
+    const float u = hh.hi - S;
+    return u + ((((hh.lo - ea) - eb) - ec) - (es + eS));
+}
 
 // Pressure, viscosity and colour-field terms of one accepted pair.
 __device__ __forceinline__ void force_pair(const float4 &pi, const float4 &ei,
                                            const float4 &pj, const float4 &ej,
-                                           const gsph_constants &c, double h2d,
+                                           const gsph_constants &c, HH hh,
                                            ForceAcc &A) {
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-    const float d2 = fmaf(rz, rz, fmaf(ry, ry, rx * rx));
+    float d2;
+    const float t = h2_minus_r2(rx, ry, rz, hh, d2);
     const float inv = __frsqrt_rn(d2);
     const float V = pj.w;
-    const float t = h2_minus_r2(rx, ry, rz, h2d);
     // h - r without cancellation: one Newton step on kr (h + r) = t from
     // h - r, with 1/(2h) for 1/(h + r); it leaves dr kr / h of the error dr of
     // r, about 2 ulp of kr over the whole range (r = d2 / sqrt(d2))
@@ -653,7 +681,7 @@ __device__ __forceinline__ float3 integrate_one(const PhysicsArgs &P, int i, con
     float a[3] = {aa.x, aa.y, aa.z};
     float x[3] = {p.x, p.y, p.z};
     const float alpha = 20000000.f, beta = 200000.f;
-    const float kern = (float)((double)c.kernel_h + 0.01);
+    const float kern = P.wall_margin;
     const int order[3] = {1, 0, 2};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -768,7 +796,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     const int tra_tiles = (tra + 63) >> 6;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
-    const double h2d = (double)P.c.kernel_h * (double)P.c.kernel_h;   // exact
+    const HH hh = exact_h2(P.c.kernel_h);
     const PairRange pr = pair_range(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
@@ -855,7 +883,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                         while (mm) {
                             const int kk = 32 * c + __builtin_ctz(mm);
                             mm &= mm - 1u;
-                            force_pair(pi, ei, sp[kk], se[kk], P.c, h2d, A);
+                            force_pair(pi, ei, sp[kk], se[kk], P.c, hh, A);
                         }
                     }
                 } else if (!self_test) {
@@ -868,7 +896,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                     while (mm) {
                         const int kk = __builtin_ctz(mm);
                         mm &= mm - 1u;
-                        force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
+                        force_pair(pi, ei, q[kk], qe[kk], P.c, hh, A);
                     }
                 } else {
                     // phase 1 tests up to 32 candidates into a mask, phase 2
@@ -889,7 +917,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                         while (mask) {
                             const int kk = b0 + __builtin_ctz(mask);
                             mask &= mask - 1u;
-                            force_pair(pi, ei, q[kk], qe[kk], P.c, h2d, A);
+                            force_pair(pi, ei, q[kk], qe[kk], P.c, hh, A);
                         }
                     }
                 }
@@ -964,7 +992,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                         while (mask) {
                             const int kk = k0 + __builtin_ctz(mask);
                             mask &= mask - 1u;
-                            force_pair(pi, ei, lp[kk], le[kk], P.c, h2d, A);
+                            force_pair(pi, ei, lp[kk], le[kk], P.c, hh, A);
                         }
                     }
                     wave_sync_lds();
@@ -1039,41 +1067,7 @@ __global__ void k_checksum_final(const double *partial, int nblocks, double *out
     out[threadIdx.x] = t;
 }
 
-// Plain float4 copy: calibration reference for HBM byte counters.
-__global__ __launch_bounds__(kNT) void k_copy(const float4 *src, float4 *dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * kNT + threadIdx.x; i < n; i += (size_t)gridDim.x * kNT)
-        dst[i] = src[i];
-}
-
-// Eight independent FP32 FMA chains per lane, 64 FMAs per iteration: the
-// peak VALU issue rate (calibration reference for the VALU counters).
-__global__ __launch_bounds__(kProbeThreads) void k_fma_probe(float *out, int iters, float b, float c) {
-    float a[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = (float)(threadIdx.x + u);
-    for (int i = 0; i < iters; ++i) {
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) a[u] = fmaf(a[u], b, c);
-        }
-    }
-    float t = 0.f;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) t += a[u];
-    out[(size_t)blockIdx.x * kProbeThreads + threadIdx.x] = t;
-}
-
 }  // namespace
-
-void launch_fma_probe(float *out, int blocks, int iters, hipStream_t s) {
-    if (blocks > 0) hipLaunchKernelGGL(k_fma_probe, dim3(blocks), dim3(kProbeThreads), 0, s, out, iters, 0.999f, 0.001f);
-}
-
-void launch_copy(const float4 *src, float4 *dst, size_t n, hipStream_t s) {
-    const int blocks = (int)std::min<size_t>((n + kNT - 1) / kNT, 8192);
-    if (blocks > 0) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kNT), 0, s, src, dst, n);
-}
 
 template <bool B, int S>
 void launch_density_t(const PhysicsArgs &p, hipStream_t s) {

@@ -158,8 +158,6 @@ _SIGS = {
     "gsph_kernel_timing": (_I, [_P, _I]),
     "gsph_kernel_times": (_I, [_P, C.c_char_p, _I, _P, _P, _I, C.POINTER(_I)]),
     "gsph_inject_fault": (_I, [_P, _I]),
-    "gsph_probe_copy": (_I, [C.c_double, _I, C.POINTER(C.c_float)]),
-    "gsph_probe_valu": (_I, [_I, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
     "gsph_server_create": (_I, [C.POINTER(Params), C.POINTER(Options), C.c_double, _I,
                                 C.POINTER(_P)]),
     "gsph_server_destroy": (_I, [_P]),
@@ -229,17 +227,49 @@ def device_count() -> int:
     return n.value
 
 
+PROBE_PATH = PKG_DIR / "_build" / "libgsph_probe.so"
+_PROBE_SIGS = {
+    "gsph_probe_last_error": (C.c_char_p, []),
+    "gsph_probe_copy": (_I, [C.c_double, _I, C.POINTER(C.c_float)]),
+    "gsph_probe_valu": (_I, [_I, C.POINTER(C.c_float), C.POINTER(C.c_double)]),
+}
+# Every symbol include/gsph_probe.h declares (checked by the CPU tests).
+PROBE_EXPORTED = sorted(_PROBE_SIGS)
+_probe = None
+
+
+def load_probe():
+    """The measurement-only library libgsph_probe.so (calibration kernels)."""
+    global _probe
+    if _probe is None:
+        if not PROBE_PATH.exists():
+            raise FileNotFoundError(f"{PROBE_PATH} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(str(PROBE_PATH))
+        for name, (res, args) in _PROBE_SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _probe = lib
+    return _probe
+
+
+def _check_probe(rc):
+    if rc != 0:
+        msg = load_probe().gsph_probe_last_error()
+        raise GsphError(rc, msg.decode() if msg else "")
+
+
 def probe_copy(nbytes: float, reps: int = 10) -> float:
     """Average ms of a plain copy kernel over nbytes (HBM counter calibration)."""
     ms = C.c_float()
-    _check(load().gsph_probe_copy(float(nbytes), int(reps), C.byref(ms)))
+    _check_probe(load_probe().gsph_probe_copy(float(nbytes), int(reps), C.byref(ms)))
     return ms.value
 
 
 def probe_valu(iters: int = 4000):
     """Peak FP32 VALU rate: (ms, lane operations) of one FMA-chain launch."""
     ms, ops = C.c_float(), C.c_double()
-    _check(load().gsph_probe_valu(int(iters), C.byref(ms), C.byref(ops)))
+    _check_probe(load_probe().gsph_probe_valu(int(iters), C.byref(ms), C.byref(ops)))
     return ms.value, ops.value
 
 

@@ -326,16 +326,6 @@ int gsph_mask_pool_info(gsph_ctx *ctx, int *used_rows, int *capacity_rows);
 int gsph_kernel_timing(gsph_ctx *ctx, int enable);
 int gsph_kernel_times(gsph_ctx *ctx, char *names, int names_cap, float *ms,
                       int *launches, int cap, int *count);
-/* Measurement helper with no counterpart in the reference: a plain float4
- * copy kernel over `bytes` bytes, run reps times (average ms returned).  Its
- * HBM byte counters calibrate the traffic figures of the other kernels. */
-int gsph_probe_copy(double bytes, int reps, float *ms);
-/* Measurement helper with no counterpart in the reference: eight independent
- * FP32 FMA chains per lane on 8 waves per SIMD, iters x 64 FMAs per lane.
- * Returns the launch's ms and its lane operations (threads x FMAs): the peak
- * VALU rate against which the density and force kernels' active-lane
- * operations (SQ_THREAD_CYCLES_VALU) are measured.  iters <= 2^20 (about 1 s). */
-int gsph_probe_valu(int iters, float *ms, double *lane_ops);
 /* Test hook: the next call that launches kernels fails with GSPH_ERR_INJECTED
  * after launching nothing (failure-path tests of the serving loop). */
 int gsph_inject_fault(gsph_ctx *ctx, int after_calls);

@@ -301,6 +301,38 @@ def test_determinism_and_prune_equivalence(gsph, po):
         assert np.array_equal(out[0][k], out[2][k])     # pruned rows change nothing
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_prune_margins_at_large_coordinates(gsph, po, axis):
+    """Row pruning at coordinates near 16000 h (grids up to 16383 cells per
+    axis): the FP32 ulp of a coordinate is about 1.5e-3 h there, above the
+    fixed prune margins, so the margins scale with the coordinate.  A
+    jittered block with cells of 8 particles: pruned and unpruned steps are
+    bitwise equal, and the density matches the O(N^2) FP64 brute force."""
+    h = 0.02
+    world = [24 * h, 24 * h, 24 * h]
+    world[axis] = 16380 * h
+    op = make_params(h=h, world=tuple(world))
+    corner = [4 * h, 4 * h, 4 * h]
+    corner[axis] = 16000 * h + 0.0037
+    pos = po.scene_block(corner, 16, 16, 16, h / 2, 0.01, 31 + axis)
+    banner(f"prune margins axis={axis} n={len(pos)}")
+    out = []
+    for prune in (1, 0):
+        s = make_system(gsph, op, pos, row_prune=prune)
+        s.arrange()
+        s.density()
+        mid = s.read_state()
+        s.force()
+        s.integrate()
+        out.append(mid + s.read_state())
+        s.close()
+    for k in range(8):
+        assert np.array_equal(out[0][k], out[1][k]), k
+    rho = po.density_brute_f64(op, out[0][0])
+    err = np.abs(out[0][0][:, 3].astype(np.float64) - 1.0 / rho) * rho
+    assert err.max() <= REL_TOL, err.max()
+
+
 ALL_TRA = {"split_mode": 1, "sms_min_self": 1 << 20, "sms_min_total": 1 << 20}
 
 
