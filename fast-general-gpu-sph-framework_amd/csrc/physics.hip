@@ -324,6 +324,49 @@ __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi,
     }
 }
 
+// The order in which every path visits a particle's 9 rows (r = 3 (dz + 1)
+// + dy + 1): mirrored pairs (0, 8), (1, 7), (2, 6), (3, 5), then the centre
+// row 4.  A particle near one face of its cell has many pairs in the rows on
+// that side and few in the mirrored ones, so a pair's total varies much less
+// from lane to lane than one row's count: the staged force walk takes a pair of
+// rows as one per-lane loop.  Density sums each row apart and adds the row
+// sums in this order; force accumulates its pairs in this order; the TRA and
+// SMS paths both follow it, so a particle gets the same bits on either path.
+__device__ __forceinline__ constexpr int row_of(int s) {
+    return (s & 1) ? 8 - (s >> 1) : (s >> 1);
+}
+
+// Union span of each row over the tile's lanes: [L[r], L[r] + S[r]) in the
+// sorted order (S = 0 when no lane has a candidate in the row).  Wave-uniform.
+__device__ __forceinline__ void row_spans(const int lo[9], const int hi[9], int L[9], int S[9]) {
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const bool live = lo[r] < hi[r];
+        const int lmin = wave_min(live ? lo[r] : kNone);
+        const int hmax = lmin == kNone ? 0 : wave_max_neg(live ? hi[r] : lmin);
+        L[r] = lmin == kNone ? 0 : lmin;
+        S[r] = lmin == kNone ? 0 : hmax - lmin;
+    }
+}
+
+// Copies src[L, L + S) into LDS at dst with LDS-direct loads
+// (global_load_lds_dwordx4: lane l of a wave writes dst + 16 l; no registers
+// are involved).  dst is wave-uniform; completion is tracked by vmcnt, so the
+// caller waits with s_waitcnt before reading dst.
+__device__ __forceinline__ void dma_span(const float4 *src, int L, int S, float4 *dst, int lane) {
+    for (int c = 0; c < S; c += 64) {
+        if (c + lane < S)
+            __builtin_amdgcn_global_load_lds((const void *)(src + L + c + lane),
+                                             (__attribute__((address_space(3))) void *)(dst + c),
+                                             16, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void wait_dma() {
+    __builtin_amdgcn_s_waitcnt(0);   // vmcnt, expcnt, lgkmcnt all zero
+    wave_sync_lds();
+}
+
 // Upper bound of a TRA tile's staging rounds (next_round below): each live
 // row's rounds start at the wave's lowest lo and advance by at least kStage
 // while some lane's range reaches past the round, so a row takes at most
@@ -344,27 +387,31 @@ __device__ __forceinline__ int tra_round_bound(const int lo[9], const int hi[9])
 }
 
 // All 9 rows of a TRA tile walked as one sequence of staging rounds (rows in
-// order; within a row, the lanes' ranges [lo, hi) are covered by rounds of
-// kStage consecutive particles loaded into LDS with coalesced loads, gaps
-// between lanes' ranges skipped; each lane reads its own candidates from LDS
-// in index order), software-pipelined by the callers:
+// the canonical order row_of(0..8); within a row, the lanes' ranges [lo, hi)
+// are covered by rounds of kStage consecutive particles loaded into LDS with
+// coalesced loads, gaps between lanes' ranges skipped; each lane reads its
+// own candidates from LDS in index order), software-pipelined by the callers:
 // while round t is visited, the loads of round t + 1 (possibly the next
 // row's first) are in flight.  Rows empty for every lane are skipped.  The
 // row bounds come from LDS (rlo/rhi: row r of lane l at r * 64 + l), so the
-// wave-uniform row index never indexes a register array.
+// wave-uniform row index never indexes a register array.  This walk serves
+// the tiles the staged-row path (stage_rows below) cannot take: rows longer
+// than its buffers, and tiles whose masks live in the pool.
 //
-// Advances (r, cur) to the next round; cur = kNone when all rows are done.
+// Advances (s, cur) to the next round (s: position in the row order);
+// cur = kNone when all rows are done.
 template <int kStage>
-__device__ __forceinline__ void next_round(int &r, int &cur, int &lo_r, int &hi_r,
+__device__ __forceinline__ void next_round(int &s, int &cur, int &lo_r, int &hi_r,
                                            const int *rlo, const int *rhi, int lane) {
     int nxt = kNone;
     // only lanes with candidates in the row steer the walk, so the rounds
     // depend on the live ranges alone (density and force must walk the same
     // rounds: the pool masks are indexed by round)
-    if (r >= 0)
+    if (s >= 0)
         nxt = wave_min(lo_r < hi_r && hi_r > cur + kStage ? max(lo_r, cur + kStage) : kNone);
-    while (nxt == kNone && r < 8) {
-        ++r;
+    while (nxt == kNone && s < 8) {
+        ++s;
+        const int r = row_of(s);
         lo_r = rlo[r * 64 + lane];
         hi_r = rhi[r * 64 + lane];
         nxt = wave_min(lo_r < hi_r ? lo_r : kNone);
@@ -395,13 +442,34 @@ __device__ __forceinline__ void store_round(float4 *dst, int lane, const float4 
     }
 }
 
+// LDS of the density kernel, per wave: the round walk's row bounds and
+// staging round, or an SMS item's batch (64 entries).
+template <int kStage>
+struct DensLds {
+    static constexpr int kWords = 2 * 9 * 64 / 4 + kStage;    // float4s: rows + round
+};
+
+// One row of a TRA lane from staged candidates q[0, cnt): per-row sum rs (a
+// rejected candidate adds an exact zero) and accept mask mk (bits in
+// candidate order enter at the bottom).
+__device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, int cnt, float h2,
+                                            PairRange pr, float &rs, uint32_t &mk) {
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
+    }
+    for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
+}
+
 template <bool kBatch, int kStage>
 __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
-    __shared__ float4 s_nb[kNT / 64][64];
-    __shared__ float4 s_stage[kNT / 64][kStage];
-    __shared__ int s_rows[kNT / 64][2][9 * 64];
+    __shared__ float4 s_lds[kNT / 64][DensLds<kStage>::kWords];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int *rlo = s_rows[wave][0], *rhi = s_rows[wave][1];
+    float4 *wl = s_lds[wave];
+    // round walk: row bounds (ints) then the staging round; SMS: 64 entries
+    int *rlo = reinterpret_cast<int *>(wl), *rhi = rlo + 9 * 64;
+    float4 *stg = wl + 2 * 9 * 64 / 4;
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
     const int tra_tiles = (tra + 63) >> 6;
@@ -409,8 +477,8 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     const ItemMap m = item_map(items);
     const float h2 = P.c.kernel_2;
     const PairRange pr = pair_range(h2);
-    float4 *stg = s_stage[wave];
     for (int it = m.first; it < m.end; it += m.stride) {
+        wave_sync_lds();   // the previous item's LDS reads precede this item's writes
         if (it < tra_tiles) {
             // all lanes stay in the wave (the row walk is cooperative);
             // lanes past the TRA range carry empty rows
@@ -420,110 +488,101 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             const float4 pi = P.pos[i];
             int lo[9], hi[9];
             tra_rows<kBatch>(P, pi, active, i, lo, hi);
-            // per-row partial sums (rejected candidates add exact zeros, so
-            // pruning cannot change the result) and per-row accept masks
+            bool lng = false;
 #pragma unroll
-            for (int r = 0; r < 9; ++r) {
-                rlo[r * 64 + lane] = lo[r];
-                rhi[r * 64 + lane] = hi[r];
-            }
-            // a tile with a row longer than one mask word keeps its masks in
-            // the pool: rows for an upper bound of its staging rounds (each
-            // row's rounds advance by at least kStage over the wave's span)
-            int pbase = -1, pool_rounds = 0;
-            if (P.use_masks) {
-                bool lng = false;
+            for (int r = 0; r < 9; ++r) lng |= active && hi[r] - lo[r] > kMaskRowMax;
+            const bool any_long = __ballot(lng) != 0;
+            float s = 0.f;
+            bool long_row = false;
+            int pbase = -1;
+            bool pool = false;
+            {
 #pragma unroll
-                for (int r = 0; r < 9; ++r) lng |= active && hi[r] - lo[r] > kMaskRowMax;
-                if (__ballot(lng) != 0) {
+                for (int r = 0; r < 9; ++r) {
+                    rlo[r * 64 + lane] = lo[r];
+                    rhi[r * 64 + lane] = hi[r];
+                }
+                // a tile with a row longer than one mask word keeps its masks in
+                // the pool: rows for an upper bound of its staging rounds (each
+                // row's rounds advance by at least kStage over the wave's span)
+                int pool_rounds = 0;
+                if (P.use_masks && any_long) {
                     pool_rounds = tra_round_bound<kStage>(lo, hi);
                     pbase = pool_take(P, pool_rounds * (kStage / 32), min(P.sms_pool_rows, kPoolBaseMax),
                                       lane);
                 }
-            }
-            const bool pool = pbase >= 0;
-            int round = 0;
-            // per-row partial sums (rejected candidates add exact zeros, so
-            // pruning cannot change the result) and per-row accept masks
-            float s = 0.f, rs = 0.f;
-            uint32_t mk = 0;
-            bool long_row = false;
-            int done = 0;   // rows [0, done) are finished
-            auto finish_rows = [&](int upto) __attribute__((always_inline)) {
-                for (; done < upto; ++done) {
-                    s += rs;
-                    rs = 0.f;
-                    if (P.use_masks && !pool) {
-                        const int cnt = rhi[done * 64 + lane] - rlo[done * 64 + lane];
-                        long_row |= cnt > kMaskRowMax;
-                        if (active)
-                            P.nbmask[(size_t)done * P.mask_stride + idx] =
-                                cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
-                    }
-                    mk = 0;
-                }
-            };
-            float4 v0 = pi, v1 = pi, v2 = pi, v3 = pi;
-            int r = -1, cur = kNone, lo_r = 0, hi_r = 0;
-            next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
-            if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
-            while (cur != kNone) {
-                const int rr = r, base = cur, lo_c = lo_r, hi_c = hi_r;
-                store_round<kStage>(stg, lane, v0, v1, v2, v3);
-                wave_sync_lds();
-                next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
-                if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
-                finish_rows(rr);
-                const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
-                const int cnt = je - jb;
-                if (pool) {
-                    // the same candidates in the same order, 32 window
-                    // positions per mask word
-#pragma unroll
-                    for (int c = 0; c < kStage / 32; ++c) {
-                        const int cb = max(jb, base + 32 * c), ce = min(je, base + 32 * c + 32);
-                        const int nc = ce - cb;
-                        uint32_t w = 0;
-                        if (nc > 0) {
-                            const float4 *q = stg + (cb - base);
-                            int k = 0;
-                            for (; k + 4 <= nc; k += 4) {
-#pragma unroll
-                                for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, w);
-                            }
-                            for (; k < nc; ++k) density_pair(pi, q[k], h2, pr, rs, w);
-                            w = finish_mask(w, nc) << (cb - base - 32 * c);
+                pool = pbase >= 0;
+                int round = 0;
+                // per-row partial sums (rejected candidates add exact zeros, so
+                // pruning cannot change the result) and per-row accept masks
+                float rs = 0.f;
+                uint32_t mk = 0;
+                int done = 0;   // rows at order positions [0, done) are finished
+                auto finish_rows = [&](int upto) __attribute__((always_inline)) {
+                    for (; done < upto; ++done) {
+                        s += rs;
+                        rs = 0.f;
+                        if (P.use_masks && !pool) {
+                            const int r = row_of(done);
+                            const int cnt = rhi[r * 64 + lane] - rlo[r * 64 + lane];
+                            long_row |= cnt > kMaskRowMax;
+                            if (active)
+                                P.nbmask[(size_t)r * P.mask_stride + idx] =
+                                    cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
                         }
-                        if (round < pool_rounds)   // always, by the bound
-                            P.sms_pool[(size_t)(pbase + round * (kStage / 32) + c) * 64 + lane] = w;
+                        mk = 0;
                     }
-                    ++round;
-                } else if (cnt > 0) {
-                    const float4 *q = stg + (jb - base);
-                    int k = 0;
-                    for (; k + 4 <= cnt; k += 4) {
+                };
+                float4 v0 = pi, v1 = pi, v2 = pi, v3 = pi;
+                int q = -1, cur = kNone, lo_r = 0, hi_r = 0;
+                next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
+                if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
+                while (cur != kNone) {
+                    const int qq = q, base = cur, lo_c = lo_r, hi_c = hi_r;
+                    store_round<kStage>(stg, lane, v0, v1, v2, v3);
+                    wave_sync_lds();
+                    next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
+                    if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
+                    finish_rows(qq);
+                    const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
+                    const int cnt = je - jb;
+                    if (pool) {
+                        // the same candidates in the same order, 32 window
+                        // positions per mask word
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
+                        for (int c = 0; c < kStage / 32; ++c) {
+                            const int cb = max(jb, base + 32 * c), ce = min(je, base + 32 * c + 32);
+                            const int nc = ce - cb;
+                            uint32_t w = 0;
+                            if (nc > 0) {
+                                density_row(pi, stg + (cb - base), nc, h2, pr, rs, w);
+                                w = finish_mask(w, nc) << (cb - base - 32 * c);
+                            }
+                            if (round < pool_rounds)   // always, by the bound
+                                P.sms_pool[(size_t)(pbase + round * (kStage / 32) + c) * 64 + lane] = w;
+                        }
+                        ++round;
+                    } else if (cnt > 0) {
+                        density_row(pi, stg + (jb - base), cnt, h2, pr, rs, mk);
                     }
-                    for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
+                    wave_sync_lds();
                 }
-                wave_sync_lds();
+                finish_rows(9);
             }
-            finish_rows(9);
             if (active) {
                 density_store(P, i, s);
                 if (P.use_masks) {
-                    bool fits = true;
+                    bool fits_plan = true;
 #pragma unroll
                     for (int r = 0; r < 9; ++r) {
                         const int len = hi[r] - lo[r], off = len > 0 ? lo[r] - i : 0;
-                        fits &= len <= 255 && off >= -(1 << 23) && off < (1 << 23);
+                        fits_plan &= len <= 255 && off >= -(1 << 23) && off < (1 << 23);
                         P.nbmask[(size_t)(10 + r) * P.mask_stride + idx] =
                             ((uint32_t)off & 0xffffffu) | ((uint32_t)len << 24);
                     }
                     P.nbmask[(size_t)9 * P.mask_stride + idx] =
-                        pool ? (kFlagPool | ((uint32_t)pbase << 3) | (fits ? kFlagPlan : 0u))
-                             : ((long_row ? kFlagLongRow : 0u) | (fits ? kFlagPlan : 0u));
+                        pool ? (kFlagPool | ((uint32_t)pbase << 3) | (fits_plan ? kFlagPlan : 0u))
+                             : ((long_row ? kFlagLongRow : 0u) | (fits_plan ? kFlagPlan : 0u));
                 }
             }
         } else {
@@ -533,7 +592,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             const SceneGeom g = P.scenes[sm.gidx];
             const int width = sm.shared ? 64 : 32;
             const int sl = sm.shared ? lane : (lane & 31);
-            float4 *lds = sm.shared ? s_nb[wave] : &s_nb[wave][lane & 32];
+            float4 *lds = sm.shared ? wl : wl + (lane & 32);
             int lo[9], hi[9], subs = 0;
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
@@ -566,7 +625,9 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             }
             float s = 0.f;
             int w = 0;
-            for (int r = 0; r < 9; ++r) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const int r = row_of(q);
                 float rs = 0.f;
                 // the next batch's candidate is loaded while this one computes
                 float4 nx = pi;
@@ -578,14 +639,8 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                     if (sl < min(width, hi[r] - b0 - width)) nx = P.pos[b0 + width + sl];
                     for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                         const int n = min(32, cnt - k0);
-                        const float4 *q = lds + k0;
                         uint32_t mk = 0;
-                        int k = 0;
-                        for (; k + 4 <= n; k += 4) {
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
-                        }
-                        for (; k < n; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
+                        density_row(pi, lds + k0, n, h2, pr, rs, mk);
                         if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_mask(mk, n);
                     }
                     wave_sync_lds();
@@ -776,14 +831,36 @@ __device__ __forceinline__ void force_store(const PhysicsArgs &P, int i, const f
     }
 }
 
+// LDS of the force kernel, per wave: the staged-row path's two group buffers
+// (pos then ev, kForceCap entries each), or the round walk's row bounds and
+// staging round (pos, ev), or an SMS item's batch.
+constexpr int kForceCap = 160;   // entries of one row pair (group) per buffer
+template <int kStage>
+struct ForceLds {
+    static constexpr int kWalk = 2 * 9 * 64 / 4 + 2 * kStage;
+    static constexpr int kWords = (4 * kForceCap > kWalk ? 4 * kForceCap : kWalk);
+};
+
+// Accept mask of cnt <= 32 staged candidates q[0, cnt): bit k for candidate k.
+__device__ __forceinline__ uint32_t test_row(const float4 &pi, const float4 *q, int cnt,
+                                             PairRange pr) {
+    uint32_t mask = 0;   // bits enter at the bottom (reversed order, undone below)
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
+    }
+    for (; k < cnt; ++k) mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
+    return finish_mask(mask, cnt);
+}
+
 template <bool kBatch, int kStage>
 __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
-    // one LDS slice per wave: staged rows (pos, ev) or SMS batches
-    __shared__ float4 s_wave[kNT / 64][2 * kStage];
-    __shared__ int s_rows[kNT / 64][2][9 * 64];
+    __shared__ float4 s_lds[kNT / 64][ForceLds<kStage>::kWords];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int *rlo = s_rows[wave][0], *rhi = s_rows[wave][1];
-    float4 *sp = s_wave[wave];
+    float4 *wl = s_lds[wave];
+    int *rlo = reinterpret_cast<int *>(wl), *rhi = rlo + 9 * 64;
+    float4 *sp = wl + 2 * 9 * 64 / 4;   // round walk: pos, then ev
     float4 *se = sp + kStage;
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
@@ -820,108 +897,152 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 tra_rows<kBatch>(P, pi, active, i, lo, hi);
             }
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            // each row is staged through LDS; lanes whose density pass
-            // recorded the row's accepted-pair mask run the pair body over
-            // it, the others (masks off, or a row longer than one mask word)
-            // test the staged candidates first.  Same pairs, same order.
-            // Tiles with long rows read their masks per staging round from
-            // the pool (lane 0 is active in every tile: its flag is the tile's).
+            // Lanes whose density pass recorded accepted-pair masks run the
+            // pair body over them; the others (masks off, or a row longer
+            // than one mask word) test the staged candidates first.  Same
+            // pairs, same order.  Tiles with long rows read their masks per
+            // staging round from the pool (lane 0 is active in every tile: its
+            // flag is the tile's).
             const uint32_t flag0 = __builtin_amdgcn_readfirstlane(flag);
             const bool pool = P.use_masks && (flag0 & kFlagPool) != 0;
-            const int pbase = (int)(flag0 >> 3);
-            const int pool_rounds = pool ? tra_round_bound<kStage>(lo, hi) : 0;   // density's bound
             const bool self_test = !P.use_masks || (active && (flag & kFlagLongRow) != 0);
             const bool use_mask = P.use_masks && active && !self_test && !pool;
+            int L[9], S[9];
+            row_spans(lo, hi, L, S);
+            bool lng = false;
 #pragma unroll
-            for (int r = 0; r < 9; ++r) {
-                rlo[r * 64 + lane] = lo[r];
-                rhi[r * 64 + lane] = hi[r];
-            }
-            float4 p0 = pi, p1 = pi, p2 = pi, p3 = pi, e0 = ei, e1 = ei, e2 = ei, e3 = ei;
-            uint32_t m_ld = 0, m_st = 0;   // row mask of the loaded / the staged round
-            int r_ld = -1;
-            int r = -1, cur = kNone, lo_r = 0, hi_r = 0;
-            uint32_t pw_ld[kStage / 32], pw_st[kStage / 32];   // pool words of a round
-            int round_ld = 0;
-            auto load = [&]() __attribute__((always_inline)) {
-                load_round<kStage>(P.pos, cur, P.n, lane, p0, p1, p2, p3);
-                load_round<kStage>(P.ev, cur, P.n, lane, e0, e1, e2, e3);
-                if (r != r_ld) {
-                    r_ld = r;
-                    m_ld = use_mask ? P.nbmask[(size_t)r * P.mask_stride + idx] : 0u;
+            for (int r = 0; r < 9; ++r) lng |= self_test && hi[r] - lo[r] > kMaskRowMax;
+            bool fits = !pool && __ballot(lng) == 0;
+#pragma unroll
+            for (int g = 0; g < 5; ++g)
+                fits = fits && S[row_of(2 * g)] + (g < 4 ? S[row_of(2 * g + 1)] : 0) <= kForceCap;
+            if (fits) {
+                // Staged-row path: the rows in mirrored pairs (groups), both
+                // rows of a group in LDS at once (pos and ev by LDS-direct
+                // loads, the next group's while this one computes), and each
+                // lane walks its accepted pairs of the two rows as one loop:
+                // row a in index order, then row b.
+                // a group's staged rows and its two mask words are loaded
+                // while the previous group computes
+                uint32_t nma = 0u, nmb = 0u;
+                auto issue = [&](int g) __attribute__((always_inline)) {
+                    float4 *bp = wl + (g & 1) * 2 * kForceCap, *be = bp + kForceCap;
+                    const int ra = row_of(2 * g), rb = g < 4 ? row_of(2 * g + 1) : 0;
+                    const int sb = g < 4 ? S[rb] : 0;
+                    dma_span(P.pos, L[ra], S[ra], bp, lane);
+                    dma_span(P.ev, L[ra], S[ra], be, lane);
+                    dma_span(P.pos, L[rb], sb, bp + S[ra], lane);
+                    dma_span(P.ev, L[rb], sb, be + S[ra], lane);
+                    nma = use_mask ? P.nbmask[(size_t)ra * P.mask_stride + idx] : 0u;
+                    nmb = use_mask && g < 4 ? P.nbmask[(size_t)rb * P.mask_stride + idx] : 0u;
+                };
+                issue(0);
+#pragma unroll
+                for (int g = 0; g < 5; ++g) {
+                    wait_dma();
+                    uint32_t ma = nma, mb = nmb;
+                    if (g + 1 < 5) issue(g + 1);
+                    const float4 *bp = wl + (g & 1) * 2 * kForceCap, *be = bp + kForceCap;
+                    const int ra = row_of(2 * g), rb = g < 4 ? row_of(2 * g + 1) : 0;
+                    const int oa = lo[ra] - L[ra];
+                    const int ob = g < 4 ? S[ra] + lo[rb] - L[rb] : 0;
+                    if (self_test) {
+                        ma = test_row(pi, bp + oa, hi[ra] - lo[ra], pr);
+                        if (g < 4) mb = test_row(pi, bp + ob, hi[rb] - lo[rb], pr);
+                    }
+                    while (ma | mb) {
+                        const bool ua = ma != 0;
+                        const int j = (ua ? oa : ob) + __builtin_ctz(ua ? ma : mb);
+                        if (ua)
+                            ma &= ma - 1u;
+                        else
+                            mb &= mb - 1u;
+                        force_pair(pi, ei, bp[j], be[j], P.c, hh, A);
+                    }
                 }
-                if (pool) {
+            } else {
+                const int pbase = (int)(flag0 >> 3);
+                const int pool_rounds = pool ? tra_round_bound<kStage>(lo, hi) : 0;   // density's bound
 #pragma unroll
-                    for (int c = 0; c < kStage / 32; ++c)
-                        pw_ld[c] = round_ld < pool_rounds
-                                       ? P.sms_pool[(size_t)(pbase + round_ld * (kStage / 32) + c) * 64 + lane]
-                                       : 0u;
-                    ++round_ld;
+                for (int r = 0; r < 9; ++r) {
+                    rlo[r * 64 + lane] = lo[r];
+                    rhi[r * 64 + lane] = hi[r];
                 }
-            };
-            next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
-            if (cur != kNone) load();
-            while (cur != kNone) {
-                const int base = cur, lo_c = lo_r, hi_c = hi_r;
-                store_round<kStage>(sp, lane, p0, p1, p2, p3);
-                store_round<kStage>(se, lane, e0, e1, e2, e3);
-                m_st = m_ld;
+                float4 p0 = pi, p1 = pi, p2 = pi, p3 = pi, e0 = ei, e1 = ei, e2 = ei, e3 = ei;
+                uint32_t m_ld = 0, m_st = 0;   // row mask of the loaded / the staged round
+                int q_ld = -1;
+                int q = -1, cur = kNone, lo_r = 0, hi_r = 0;
+                uint32_t pw_ld[kStage / 32], pw_st[kStage / 32];   // pool words of a round
+                int round_ld = 0;
+                auto load = [&]() __attribute__((always_inline)) {
+                    load_round<kStage>(P.pos, cur, P.n, lane, p0, p1, p2, p3);
+                    load_round<kStage>(P.ev, cur, P.n, lane, e0, e1, e2, e3);
+                    if (q != q_ld) {
+                        q_ld = q;
+                        m_ld = use_mask ? P.nbmask[(size_t)row_of(q) * P.mask_stride + idx] : 0u;
+                    }
+                    if (pool) {
 #pragma unroll
-                for (int c = 0; c < kStage / 32; ++c) pw_st[c] = pool ? pw_ld[c] : 0u;
-                wave_sync_lds();
-                next_round<kStage>(r, cur, lo_r, hi_r, rlo, rhi, lane);
+                        for (int c = 0; c < kStage / 32; ++c)
+                            pw_ld[c] = round_ld < pool_rounds
+                                           ? P.sms_pool[(size_t)(pbase + round_ld * (kStage / 32) + c) * 64 + lane]
+                                           : 0u;
+                        ++round_ld;
+                    }
+                };
+                next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
                 if (cur != kNone) load();
-                const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
-                const float4 *q = sp + (jb - base);
-                const float4 *qe = se + (jb - base);
-                if (pool) {
-                    // bit k of word c: window position 32 c + k (zero outside
-                    // this lane's range)
+                while (cur != kNone) {
+                    const int base = cur, lo_c = lo_r, hi_c = hi_r;
+                    store_round<kStage>(sp, lane, p0, p1, p2, p3);
+                    store_round<kStage>(se, lane, e0, e1, e2, e3);
+                    m_st = m_ld;
 #pragma unroll
-                    for (int c = 0; c < kStage / 32; ++c) {
-                        uint32_t mm = pw_st[c];
+                    for (int c = 0; c < kStage / 32; ++c) pw_st[c] = pool ? pw_ld[c] : 0u;
+                    wave_sync_lds();
+                    next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
+                    if (cur != kNone) load();
+                    const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
+                    const float4 *qp = sp + (jb - base);
+                    const float4 *qe = se + (jb - base);
+                    if (pool) {
+                        // bit k of word c: window position 32 c + k (zero outside
+                        // this lane's range)
+#pragma unroll
+                        for (int c = 0; c < kStage / 32; ++c) {
+                            uint32_t mm = pw_st[c];
+                            while (mm) {
+                                const int kk = 32 * c + __builtin_ctz(mm);
+                                mm &= mm - 1u;
+                                force_pair(pi, ei, sp[kk], se[kk], P.c, hh, A);
+                            }
+                        }
+                    } else if (!self_test) {
+                        const int cnt = je - jb;   // <= 32, and jb - lo < 32
+                        uint32_t mm = 0;
+                        if (cnt > 0) {
+                            mm = m_st >> (jb - lo_c);
+                            if (cnt < 32) mm &= (1u << cnt) - 1u;
+                        }
                         while (mm) {
-                            const int kk = 32 * c + __builtin_ctz(mm);
+                            const int kk = __builtin_ctz(mm);
                             mm &= mm - 1u;
-                            force_pair(pi, ei, sp[kk], se[kk], P.c, hh, A);
+                            force_pair(pi, ei, qp[kk], qe[kk], P.c, hh, A);
+                        }
+                    } else {
+                        // 32 candidates at a time: accept mask, then the pair
+                        // body over the accepted ones (in order)
+                        for (int b0 = 0; b0 < je - jb; b0 += 32) {
+                            uint32_t mask = test_row(pi, qp + b0, min(32, je - jb - b0), pr);
+                            while (mask) {
+                                const int kk = b0 + __builtin_ctz(mask);
+                                mask &= mask - 1u;
+                                force_pair(pi, ei, qp[kk], qe[kk], P.c, hh, A);
+                            }
                         }
                     }
-                } else if (!self_test) {
-                    const int cnt = je - jb;   // <= 32, and jb - lo < 32
-                    uint32_t mm = 0;
-                    if (cnt > 0) {
-                        mm = m_st >> (jb - lo_c);
-                        if (cnt < 32) mm &= (1u << cnt) - 1u;
-                    }
-                    while (mm) {
-                        const int kk = __builtin_ctz(mm);
-                        mm &= mm - 1u;
-                        force_pair(pi, ei, q[kk], qe[kk], P.c, hh, A);
-                    }
-                } else {
-                    // phase 1 tests up to 32 candidates into a mask, phase 2
-                    // runs the pair body over the accepted ones (in order)
-                    for (int b0 = 0; b0 < je - jb; b0 += 32) {
-                        const int cnt = min(32, je - jb - b0);
-                        // bits enter at the bottom (reversed order, undone below)
-                        uint32_t mask = 0;
-                        int k = 0;
-                        for (; k + 4 <= cnt; k += 4) {
-#pragma unroll
-                            for (int u = 0; u < 4; ++u)
-                                mask += mask + (in_range(dist2(pi, q[b0 + k + u]), pr) ? 1u : 0u);
-                        }
-                        for (; k < cnt; ++k)
-                            mask += mask + (in_range(dist2(pi, q[b0 + k]), pr) ? 1u : 0u);
-                        mask = __builtin_bitreverse32(mask) >> (32 - cnt);
-                        while (mask) {
-                            const int kk = b0 + __builtin_ctz(mask);
-                            mask &= mask - 1u;
-                            force_pair(pi, ei, q[kk], qe[kk], P.c, hh, A);
-                        }
-                    }
+                    wave_sync_lds();
                 }
-                wave_sync_lds();
             }
             if (active) force_store<kBatch>(P, i, pi, ei, A);
         } else {
@@ -935,10 +1056,11 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             const SceneGeom g = P.scenes[sm.gidx];
             const int width = sm.shared ? 64 : 32;
             const int sl = sm.shared ? lane : (lane & 31);
-            float4 *lp = sm.shared ? s_wave[wave] : s_wave[wave] + (lane & 32);
+            float4 *lp = sm.shared ? wl : wl + (lane & 32);
             float4 *le = lp + 64;
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int r = 0; r < 9; ++r) {
+            for (int q = 0; q < 9; ++q) {
+                const int r = row_of(q);
                 const int ny = sm.cy + r % 3 - 1, nz = sm.cz + r / 3 - 1;
                 int lo = 0, hi = 0;
                 if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
@@ -973,22 +1095,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                     // 32 candidates at a time: accept mask, then the pair body
                     for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                         const int n = min(32, cnt - k0);
-                        const float4 *q = lp + k0;
-                        uint32_t mask = 0;
-                        if (base >= 0) {
-                            mask = k0 == 0 ? cm0 : cm1;
-                        } else {
-                            // bits enter at the bottom, reversed below
-                            int k = 0;
-                            for (; k + 4 <= n; k += 4) {
-#pragma unroll
-                                for (int u = 0; u < 4; ++u)
-                                    mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
-                            }
-                            for (; k < n; ++k)
-                                mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
-                            mask = __builtin_bitreverse32(mask) >> (32 - n);
-                        }
+                        uint32_t mask = base >= 0 ? (k0 == 0 ? cm0 : cm1) : test_row(pi, lp + k0, n, pr);
                         while (mask) {
                             const int kk = k0 + __builtin_ctz(mask);
                             mask &= mask - 1u;
