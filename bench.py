@@ -162,36 +162,61 @@ def reference_scene_leg(gsph, steps=20, warmup=3, no_autotune=False):
     }
 
 
-def serving_leg(gsph, capacity_ps, requests=96, seed=20261207, budget=200e9):
-    """Workload 4: Poisson arrivals, N log-uniform in [8192, 1048576], steps
-    uniform in 50..500, 200 GB budget; the arrival rate puts the offered load
-    at 0.7 of the measured batched capacity."""
+def serving_leg(gsph, requests=96, seed=20261207, budget=200e9, load=0.7):
+    """Workload 4: requests with N log-uniform in [8192, 1048576] and steps
+    uniform in 50..500, 200 GB HBM budget.  Three runs of the same requests:
+    (1) served capacity at saturation (every request queued at t = 0); (2)
+    latency under Poisson arrivals whose offered load is `load` x that
+    capacity; (3) the plain loop: each request alone in its own context, one
+    after the other."""
     import numpy as np
 
-    rng = np.random.default_rng(seed)
-    ns = np.exp(rng.uniform(np.log(8192), np.log(1048576), requests))
-    steps = rng.integers(50, 501, requests)
-    mean_work = float(np.mean(ns) * np.mean(steps))
-    lam = 0.7 * capacity_ps / mean_work
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
+    worlds, blocks, steps, rng = gsph.serving_workload(requests, seed)
+    ns = np.array([len(b) for b in blocks], np.float64)
+    work = float(np.sum(ns * np.array(steps)))
     params, _ = gsph.scene_config(0)
-    srv = gsph.Server(params, hbm_budget_bytes=budget, max_scenes=4096)
-    for k in range(requests):
-        side = max(8, int(round(ns[k] ** (1 / 3))))
-        pos = gsph.scene_block([0.06, 0.06, 0.06], side, side, side, 0.01, 0.01, seed + k)
-        w = 2 * side * 0.01
-        srv.submit([w, w, w], pos, int(steps[k]), float(arrivals[k]))
-    srv.run(realtime=True)
-    st = srv.stats()
-    srv.close()
+
+    def serve(arrivals):
+        srv = gsph.Server(params, hbm_budget_bytes=budget, max_scenes=4096)
+        for w, blk, st, a in zip(worlds, blocks, steps, arrivals):
+            srv.submit(w, blk, st, float(a))
+        srv.run(realtime=True)
+        st = srv.stats()
+        srv.close()
+        return st
+
+    sat = serve(np.zeros(requests))
+    capacity = sat["particle_steps"] / sat["wall"]
+    print(f"[bench] serving capacity {capacity / 1e9:.2f} G particle-steps/s", file=sys.stderr,
+          flush=True)
+    lam = load * capacity / (work / requests)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
+    lat = serve(arrivals)
+    # plain loop: one plain context per request, back to back
+    t0 = time.perf_counter()
+    for w, blk, st in zip(worlds, blocks, steps):
+        p, _ = gsph.scene_config(0)
+        for a in range(3):
+            p.world_size[a] = w[a]
+        s = gsph.System(p, blk)
+        s.step(st)
+        s.sync()
+        s.close()
+    loop_ps = work / (time.perf_counter() - t0)
     return {
-        "workload": f"{requests} requests, Poisson rate {lam:.2f}/s (0.7 x batched capacity), "
-                    "N log-uniform 8192..1048576, steps 50..500, 200 GB HBM budget",
-        "completed": st["completed"], "failed": st["failed"], "rejected": st["rejected"],
-        "p50_latency_s": st["p50"], "p99_latency_s": st["p99"],
-        "particle_steps_per_s": st["particle_steps"] / st["wall"], "wall_s": st["wall"],
-        "max_resident": st["max_active"], "recompositions": st["recompositions"],
-        "engine_steps": st["engine_steps"],
+        "workload": f"{requests} requests, N log-uniform 8192..1048576 (total {int(ns.sum())}), "
+                    f"steps 50..500, {budget / 1e9:.0f} GB HBM budget",
+        "capacity_particle_steps_per_s": capacity,
+        "capacity_wall_s": sat["wall"], "capacity_recompositions": sat["recompositions"],
+        "capacity_max_resident": sat["max_active"], "capacity_peak_bytes": sat["peak_bytes"],
+        "plain_loop_particle_steps_per_s": loop_ps,
+        "capacity_vs_plain_loop": capacity / loop_ps,
+        "load": load, "poisson_rate_per_s": lam,
+        "completed": lat["completed"], "failed": lat["failed"], "rejected": lat["rejected"],
+        "p50_latency_s": lat["p50"], "p99_latency_s": lat["p99"],
+        "particle_steps_per_s": lat["particle_steps"] / lat["wall"], "wall_s": lat["wall"],
+        "max_resident": lat["max_active"], "recompositions": lat["recompositions"],
+        "engine_steps": lat["engine_steps"],
     }
 
 
@@ -346,8 +371,10 @@ def main():
               file=sys.stderr, flush=True)
     if "serving" in legs:
         print("[bench] serving leg", file=sys.stderr, flush=True)
-        cap = out.get("batched", {}).get("particle_steps_per_s", value)
-        out["serving"] = serving_leg(gsph, cap)
+        out["serving"] = serving_leg(gsph)
+        print(f"[bench] serving capacity {out['serving']['capacity_vs_plain_loop']:.2f}x the plain "
+              f"loop; p99 {out['serving']['p99_latency_s']:.3f} s at 0.7 load", file=sys.stderr,
+              flush=True)
     if "reference" in legs:
         print("[bench] reference scene leg", file=sys.stderr, flush=True)
         out["reference_scene"] = reference_scene_leg(gsph, no_autotune=args.no_autotune)

@@ -1131,7 +1131,65 @@ __global__ void k_empty_grid(ArrangeArgs a) {
     }
 }
 
+// Serving re-layout (see launch_relayout): thread t moves destination slot t,
+// whose segment is found by binary search over the segments' dst.
+__global__ __launch_bounds__(kNT) void k_relayout(const float4 *pos_s, const f3 *vel_s,
+                                                  const float4 *ev_s, const uint32_t *hash_s,
+                                                  float4 *pos_d, f3 *vel_d, float4 *ev_d,
+                                                  uint32_t *hash_d, const RelayoutSeg *segs,
+                                                  int nseg, int total) {
+    const int t = blockIdx.x * kNT + threadIdx.x;
+    if (t >= total) return;
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (segs[m].dst <= t)
+            lo = m;
+        else
+            hi = m - 1;
+    }
+    const RelayoutSeg g = segs[lo];
+    const int src = g.src + (t - g.dst);
+    pos_d[t] = pos_s[src];
+    vel_d[t] = vel_s[src];
+    ev_d[t] = ev_s[src];
+    hash_d[t] = hash_s[src] + (uint32_t)g.key_delta;
+}
+
+// Moves the device error flags into host memory the GPU can write (pinned)
+// and clears them: no copy-engine operation between the serving loop's steps.
+__global__ void k_take_flags(int *dc, int *out) {
+    if (threadIdx.x == 0) {
+        out[0] = dc[kCntErr];
+        dc[kCntErr] = 0;
+    }
+}
+
+__global__ void k_set_order(int *dc, int order, int keys_ready) {
+    if (threadIdx.x == 0) {
+        dc[kCntOrder] = order;
+        dc[kCntKeysReady] = keys_ready;
+    }
+}
+
 }  // namespace
+
+void launch_relayout(const float4 *pos_s, const f3 *vel_s, const float4 *ev_s,
+                     const uint32_t *hash_s, float4 *pos_d, f3 *vel_d, float4 *ev_d,
+                     uint32_t *hash_d, const RelayoutSeg *segs, int nseg, int total,
+                     hipStream_t s) {
+    if (total > 0 && nseg > 0)
+        hipLaunchKernelGGL(k_relayout, dim3((total + kNT - 1) / kNT), dim3(kNT), 0, s, pos_s, vel_s,
+                           ev_s, hash_s, pos_d, vel_d, ev_d, hash_d, segs, nseg, total);
+}
+
+void launch_take_flags(int *dc, int *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_take_flags, dim3(1), dim3(64), 0, s, dc, out);
+}
+
+void launch_set_order(int *dc, int order, int keys_ready, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_order, dim3(1), dim3(64), 0, s, dc, order, keys_ready);
+}
 
 void launch_hash(const ArrangeArgs &a, bool batch, hipStream_t s) {
     if (batch)

@@ -78,6 +78,17 @@ struct gsph_ctx {
     SceneGeom *scenes_d = nullptr;
     double *sum_partial = nullptr, *sum_out = nullptr;
     std::vector<void *> allocs;
+    size_t alloc_bytes = 0;       // device bytes of the buffers above
+    RelayoutSeg *segs = nullptr;  // [cap_s] scene moves of a serving re-layout
+    // pinned host staging of asynchronous layout uploads (serving): a ring of
+    // slots, each reused only once the event of its last upload has completed
+    struct Staging {
+        void *host = nullptr;
+        size_t bytes = 0;
+        hipEvent_t done = nullptr;
+    };
+    Staging staging[4];
+    int staging_next = 0;
     // state tracking
     int cur = 0;              // buffer holding the current state (0 = A, 1 = B)
     bool grid_valid = false;  // arrange outputs match the state in B
@@ -124,6 +135,7 @@ int dalloc(gsph_ctx *ctx, T **p, size_t count) {
         return fail(ctx, GSPH_ERR_HIP, "hipMalloc(" + std::to_string(bytes) +
                                            " B): " + hipGetErrorString(e));
     ctx->allocs.push_back(q);
+    ctx->alloc_bytes += bytes;
     *p = static_cast<T *>(q);
     return GSPH_OK;
 }
@@ -447,6 +459,51 @@ int configure_layout(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n) {
     return GSPH_OK;
 }
 
+// SMS / long-row mask pool rows for a particle capacity: about one
+// 32-candidate sub-batch row (64 lanes x 4 B) per 256 bytes of
+// sms_mask_bytes per particle.
+size_t pool_rows_for(const gsph_options &o, size_t N) {
+    const size_t rows = o.sms_mask_bytes > 0 ? N * (size_t)o.sms_mask_bytes / 256 : 0;
+    return std::min<size_t>(rows, (size_t)INT32_MAX / 2);
+}
+
+}  // namespace
+
+namespace gsph {
+// Device bytes that alloc_buffers allocates for capacities N particles, C
+// cells and S scenes under options o (every buffer, as sized there; each
+// allocation at least one element).  alloc_buffers checks its own total
+// against this, so the serving loop's HBM accounting is exact.
+size_t buffer_bytes(const gsph_options &o, size_t N, size_t C, size_t S) {
+    S = std::max<size_t>(S, 1);
+    const size_t sort_tiles = std::max<size_t>(1, (N + kSortTile - 1) / kSortTile);
+    const size_t cell_tiles = std::max<size_t>(1, (N + kCellTile - 1) / kCellTile);
+    const size_t split_tiles = std::max<size_t>(1, (N + kSplitTile - 1) / kSplitTile);
+    const size_t nchunks = std::max<size_t>(1, (C + kFillChunk - 1) / kFillChunk);
+    const size_t task_cap = (N + 31) / 32 + std::min(C, N) + 2;
+    const size_t rows = pool_rows_for(o, N);
+    auto b = [](size_t count, size_t elem) { return std::max<size_t>(count, 1) * elem; };
+    size_t t = 0;
+    t += 2 * (b(N, 16) + b(N, 12) + b(N, 16) + b(N, 4) + b(N, 4));   // pos vel ev mkey mval
+    t += 4 * b(N, 4);                                                 // hash perm skey sidx
+    t += b((N + 63) / 64, 8) + b((N + 63) / 64, 4);                   // mflag mpre
+    t += 2 * b(sort_tiles, 4) + b(4 * 256, 4);                        // tile_m tile_occ dhist
+    t += b(4 * 256 * (size_t)msort_tiles((int)N), 8);                 // status
+    t += b(N, 12);                                                    // acc
+    t += b(cell_tiles, 4) + b(3 * split_tiles, 4);                    // bcnt split_sum
+    t += 2 * b(N + 1, 4) + b(nchunks + 1, 4);                         // cells cstart chunk_k0
+    t += b(C + 1, 4) + 3 * b(C, 4) + b(C, 8);                         // begin req tra sms xcum
+    t += b(N, 4) + b((size_t)kMaskWords * N, 4);                      // compact nbmask
+    t += b(rows * 64, 4) + b(task_cap / 2 + 2, 4);                    // sms_pool sms_off
+    t += b(task_cap, sizeof(gsph_task)) + b(kCntSlots, 4);            // tasks dc
+    t += b(S, sizeof(SceneGeom)) + b(S, sizeof(RelayoutSeg));         // scenes segs
+    t += b(5 * 256, 8) + b(5, 8);                                     // checksum
+    return t;
+}
+}  // namespace gsph
+
+namespace {
+
 // All device buffers, sized by capacity (particles, cells, scenes).
 int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ctx->cap_n = cap_n;
@@ -496,19 +553,18 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->nbmask, (size_t)kMaskWords * N);
     // SMS masks: about one 32-candidate sub-batch row per 8 bytes of budget
     // per particle (rows hold 64 lanes x 4 B)
-    {
-        const size_t rows = ctx->opt.sms_mask_bytes > 0
-                                ? N * (size_t)ctx->opt.sms_mask_bytes / 256 : 0;
-        ctx->sms_pool_rows = (int)std::min<size_t>(rows, (size_t)INT32_MAX / 2);
-        ALLOC(ctx->sms_pool, (size_t)ctx->sms_pool_rows * 64);
-        ALLOC(ctx->sms_off, task_cap / 2 + 2);
-    }
+    ctx->sms_pool_rows = (int)pool_rows_for(ctx->opt, N);
+    ALLOC(ctx->sms_pool, (size_t)ctx->sms_pool_rows * 64);
+    ALLOC(ctx->sms_off, task_cap / 2 + 2);
     ALLOC(ctx->tasks, task_cap);
     ALLOC(ctx->dc, kCntSlots);
     ALLOC(ctx->scenes_d, ctx->cap_s);
+    ALLOC(ctx->segs, ctx->cap_s);
     ALLOC(ctx->sum_partial, 5 * 256);
     ALLOC(ctx->sum_out, 5);
 #undef ALLOC
+    if (ctx->alloc_bytes != buffer_bytes(ctx->opt, N, C, (size_t)ctx->cap_s))
+        return fail(ctx, GSPH_ERR_HIP, "internal: buffer accounting differs from the allocation");
     HIP_TRY(ctx, hipMemsetAsync(ctx->dc, 0, sizeof(int) * kCntSlots, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->dhist, 0, sizeof(uint32_t) * 4 * 256, ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->status, 0,
@@ -668,16 +724,93 @@ int capture_graph(gsph_ctx *ctx) {
 
 // ---------------------------------------------------------------------------
 // Engine hooks of the serving loop (engine.h).
+namespace {
+
+// A pinned staging slot with at least `bytes` bytes whose previous upload has
+// completed.
+void *staging_slot(gsph_ctx *ctx, size_t bytes, hipEvent_t *done) {
+    gsph_ctx::Staging &st = ctx->staging[ctx->staging_next];
+    ctx->staging_next = (ctx->staging_next + 1) % 4;
+    if (st.done) {
+        if (hipEventSynchronize(st.done) != hipSuccess) return nullptr;
+    } else if (hipEventCreateWithFlags(&st.done, hipEventDisableTiming) != hipSuccess) {
+        st.done = nullptr;
+        return nullptr;
+    }
+    if (st.bytes < bytes) {
+        if (st.host) (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.bytes = 0;
+        const size_t want = std::max<size_t>(bytes, 4096);
+        if (hipHostMalloc(&st.host, want, hipHostMallocDefault) != hipSuccess) {
+            st.host = nullptr;
+            return nullptr;
+        }
+        st.bytes = want;
+    }
+    *done = st.done;
+    return st.host;
+}
+
+// configure_layout without a host synchronisation: the scene table (and the
+// re-layout segments) go up from a pinned staging slot.
+int configure_layout_async(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n,
+                           const std::vector<RelayoutSeg> &segs) {
+    long long numc = 0, pb = 0;
+    for (auto &g : scenes) {
+        if (g.gx <= 0 || g.gy <= 0 || g.gz <= 0 || g.gx > 16383 || g.gy > 16383 || g.gz > 16383)
+            return fail(ctx, GSPH_ERR_INVALID_ARG, "grid dimensions must be in [1, 16383]");
+        g.cell_base = (int)numc;
+        g.p_begin = (int)pb;
+        numc += (long long)g.gx * g.gy * g.gz;
+        pb += g.p_count;
+    }
+    if (numc > (1LL << 26))
+        return fail(ctx, GSPH_ERR_CAPACITY, "more than 2^26 grid cells (micro ids must fit 32 bits)");
+    if (pb != n) return fail(ctx, GSPH_ERR_INVALID_ARG, "scene counts do not add up");
+    if (n > ctx->cap_n || numc > ctx->cap_c || (int)scenes.size() > ctx->cap_s ||
+        (int)segs.size() > ctx->cap_s)
+        return fail(ctx, GSPH_ERR_CAPACITY, "layout exceeds the context capacity");
+    const size_t gb = sizeof(SceneGeom) * scenes.size(), sb = sizeof(RelayoutSeg) * segs.size();
+    if (gb + sb > 0) {
+        hipEvent_t done = nullptr;
+        char *host = static_cast<char *>(staging_slot(ctx, gb + sb, &done));
+        if (!host) return fail(ctx, GSPH_ERR_HIP, "pinned staging allocation failed");
+        if (gb) std::memcpy(host, scenes.data(), gb);
+        if (sb) std::memcpy(host + gb, segs.data(), sb);
+        if (gb) HIP_TRY(ctx, hipMemcpyAsync(ctx->scenes_d, host, gb, hipMemcpyHostToDevice, ctx->stream));
+        if (sb) HIP_TRY(ctx, hipMemcpyAsync(ctx->segs, host + gb, sb, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(done, ctx->stream));
+    }
+    ctx->scenes = scenes;
+    ctx->n = n;
+    ctx->numc = (int)numc;
+    ctx->passes = (bits_for(std::max(numc * 64 - 1, 1LL)) + 7) / 8;
+    ctx->sort_tiles = std::max(1, (n + kSortTile - 1) / kSortTile);
+    ctx->cell_tiles = std::max(1, (n + kCellTile - 1) / kCellTile);
+    ctx->nchunks = std::max(1, (ctx->numc + kFillChunk - 1) / kFillChunk);
+    ctx->split_tiles = std::max(1, (n + kSplitTile - 1) / kSplitTile);
+    ctx->task_cap = (n + 31) / 32 + std::min(ctx->numc, n) + 2;
+    if (ctx->graph_exec) {
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+    }
+    if (ctx->graph) {
+        (void)hipGraphDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    ctx->grid_valid = false;
+    ctx->masks_valid = false;
+    return GSPH_OK;
+}
+
+}  // namespace
+
 namespace gsph {
 
-size_t engine_bytes(long long n, long long cells) {
-    // per particle: 2 x state (44 B), acc, sorted keys/perm, stayer and mover
-    // lists, cell list, compact, task slot, pair masks, sort look-back words
-    // (+ the SMS mask budget); per cell: begin, req, TRA/SMS bases, x-layer
-    // starts
-    gsph_options o;
-    gsph_options_default(&o);
-    return (size_t)(n * (296 + (long long)o.sms_mask_bytes) + cells * 24 + (1 << 20));
+size_t engine_bytes(const gsph_options &o, long long cap_n, long long cap_c, long long cap_s) {
+    return buffer_bytes(o, (size_t)std::max(cap_n, 0LL), (size_t)std::max(cap_c, 0LL),
+                        (size_t)std::max(cap_s, 1LL));
 }
 
 int engine_create(const gsph_params *p, const gsph_options *opt, int cap_n, int cap_c,
@@ -702,61 +835,146 @@ void engine_capacity(const gsph_ctx *ctx, int *cap_n, int *cap_c, int *cap_s) {
     *cap_s = ctx->cap_s;
 }
 
-SceneGeom engine_geom(const gsph_ctx *ctx, const float world[3], int count) {
-    return geom_for(ctx->c, world, 0, count);
+size_t engine_allocated(const gsph_ctx *ctx) { return ctx->alloc_bytes; }
+
+long long engine_cells(const gsph_ctx *ctx, const float world[3]) {
+    const SceneGeom g = geom_for(ctx->c, world, 0, 0);
+    return (long long)g.gx * g.gy * g.gz;
 }
 
-// New scene set.  Scene k keeps its state from slot range [old_begin, +count)
-// of the current resident buffer, or (old_begin < 0) is uploaded from host
-// arrays with ev = 0.  The new layout is staged in the sorted buffer and then
-// becomes the resident one.  `dst` may be a different, larger context: the
-// state is then copied across and the caller replaces the old context.
-int engine_recompose(gsph_ctx *src, gsph_ctx *dst, const std::vector<EngineScene> &scenes) {
+hipStream_t engine_stream(const gsph_ctx *ctx) { return ctx->stream; }
+
+// Order-preserving re-layout.  Continuing scenes (old_begin >= 0) come first
+// and in their current relative order: their slots are moved (one kernel)
+// with their sorted keys shifted by the change of their first cell id, so the
+// moved keys stay sorted and the next arrange sorts only the particles that
+// change micro cell, as after any step.  New scenes follow; their state goes
+// up from pinned host memory with keys that match no micro id (all movers).
+// Nothing waits on the host unless dst != src (a grown engine), where the
+// copy finishes before returning so that src can be destroyed.
+int engine_relayout(gsph_ctx *src, gsph_ctx *dst, const std::vector<EngineScene> &scenes) {
     std::vector<SceneGeom> geo;
-    long long n = 0;
+    std::vector<RelayoutSeg> segs;
+    long long n = 0, cells = 0;
+    int last_old = -1;
+    bool new_seen = false;
     for (const auto &e : scenes) {
-        geo.push_back(geom_for(dst->c, e.world, (int)n, e.count));
+        SceneGeom g = geom_for(dst->c, e.world, (int)n, e.count);
+        const long long gc = (long long)g.gx * g.gy * g.gz;
+        if (e.old_begin >= 0) {
+            if (new_seen || e.old_begin < last_old || !src)
+                return fail(dst, GSPH_ERR_INVALID_ARG, "continuing scenes must come first, in order");
+            last_old = e.old_begin;
+            if (e.count > 0) {
+                RelayoutSeg sg;
+                sg.src = e.old_begin;
+                sg.dst = (int)n;
+                sg.count = e.count;
+                sg.key_delta = (int)((cells - (long long)e.old_cell_base) * 64);
+                segs.push_back(sg);
+            }
+        } else {
+            new_seen = true;
+        }
+        geo.push_back(g);
         n += e.count;
+        cells += gc;
     }
-    if (n > dst->cap_n) return fail(dst, GSPH_ERR_CAPACITY, "engine particle capacity");
-    if (src && src->cur != 0) return fail(dst, GSPH_ERR_INVALID_ARG, "recompose needs integrated state");
+    if (n > dst->cap_n || cells > dst->cap_c || (long long)scenes.size() > dst->cap_s)
+        return fail(dst, GSPH_ERR_CAPACITY, "engine capacity");
+    if (src && src->cur != 0) return fail(dst, GSPH_ERR_INVALID_ARG, "re-layout needs integrated state");
     hipStream_t st = dst->stream;
-    if (src && src != dst) HIP_TRY(dst, hipStreamSynchronize(src->stream));
+    if (src && src != dst) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(dst, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        hipError_t er = hipEventRecord(e, src->stream);
+        if (er == hipSuccess) er = hipStreamWaitEvent(st, e, 0);
+        (void)hipEventDestroy(e);
+        if (er != hipSuccess) return fail(dst, GSPH_ERR_HIP, std::string("re-layout: ") + hipGetErrorString(er));
+    }
+    // the scene table and segments first (the kernel reads the segments)
+    int rc = configure_layout_async(dst, geo, (int)n, segs);
+    if (rc) return rc;
+    long long cont = 0;
+    for (const auto &sg : segs) cont = std::max<long long>(cont, (long long)sg.dst + sg.count);
+    if (src)
+        launch_relayout(src->pos[0], src->vel[0], src->ev[0], src->hash, dst->pos[1], dst->vel[1],
+                        dst->ev[1], dst->skey, dst->segs, (int)segs.size(), (int)cont, st);
+    rc = check_launch(dst, "relayout");
+    if (rc) return rc;
     long long off = 0;
     for (const auto &e : scenes) {
         const size_t c = (size_t)e.count;
-        if (c == 0) continue;
-        if (e.old_begin >= 0) {
-            HIP_TRY(dst, hipMemcpyAsync(dst->pos[1] + off, src->pos[0] + e.old_begin, 16 * c,
-                                        hipMemcpyDeviceToDevice, st));
-            HIP_TRY(dst, hipMemcpyAsync(dst->vel[1] + off, src->vel[0] + e.old_begin, 12 * c,
-                                        hipMemcpyDeviceToDevice, st));
-            HIP_TRY(dst, hipMemcpyAsync(dst->ev[1] + off, src->ev[0] + e.old_begin, 16 * c,
-                                        hipMemcpyDeviceToDevice, st));
-        } else {
-            HIP_TRY(dst, hipMemcpyAsync(dst->pos[1] + off, e.pos4, 16 * c,
-                                        hipMemcpyHostToDevice, st));
+        if (c && e.old_begin < 0) {
+            HIP_TRY(dst, hipMemcpyAsync(dst->pos[1] + off, e.pos4, 16 * c, hipMemcpyHostToDevice, st));
             if (e.vel3)
-                HIP_TRY(dst, hipMemcpyAsync(dst->vel[1] + off, e.vel3, 12 * c,
-                                            hipMemcpyHostToDevice, st));
+                HIP_TRY(dst, hipMemcpyAsync(dst->vel[1] + off, e.vel3, 12 * c, hipMemcpyHostToDevice, st));
             else
                 HIP_TRY(dst, hipMemsetAsync(dst->vel[1] + off, 0, 12 * c, st));
             HIP_TRY(dst, hipMemsetAsync(dst->ev[1] + off, 0, 16 * c, st));
+            HIP_TRY(dst, hipMemsetAsync(dst->skey + off, 0xff, 4 * c, st));
         }
         off += (long long)c;
     }
-    HIP_TRY(dst, hipStreamSynchronize(st));
     std::swap(dst->pos[0], dst->pos[1]);
     std::swap(dst->vel[0], dst->vel[1]);
     std::swap(dst->ev[0], dst->ev[1]);
-    int rc = configure_layout(dst, geo, (int)n);
-    if (!rc) rc = invalidate_order(dst);
+    std::swap(dst->hash, dst->skey);
+    // the slot order is the moved keys' sorted order; the fused integration's
+    // keys are for the old slots
+    launch_set_order(dst->dc, 1, 0, st);
+    rc = check_launch(dst, "relayout order");
+    if (rc) return rc;
     dst->cur = 0;
-    return rc;
+    if (src && src != dst) HIP_TRY(dst, hipStreamSynchronize(st));
+    return GSPH_OK;
 }
 
-int engine_scene_begin(const gsph_ctx *ctx, int s) {
-    return (s >= 0 && s < (int)ctx->scenes.size()) ? ctx->scenes[s].p_begin : -1;
+int engine_scene_layout(const gsph_ctx *ctx, int s, int *first, int *cell_base) {
+    if (s < 0 || s >= (int)ctx->scenes.size()) return GSPH_ERR_INVALID_ARG;
+    *first = ctx->scenes[s].p_begin;
+    *cell_base = ctx->scenes[s].cell_base;
+    return GSPH_OK;
+}
+
+// Enqueues the copy of scene s's state (current slot order) into host memory
+// (pinned for an asynchronous copy) and records `done` after it.
+int engine_retire_async(gsph_ctx *ctx, int s, float *pos4, float *vel3, float *ev4,
+                        hipEvent_t done) {
+    if (s < 0 || s >= (int)ctx->scenes.size()) return fail(ctx, GSPH_ERR_INVALID_ARG, "scene index");
+    if (ctx->cur != 0) return fail(ctx, GSPH_ERR_INVALID_ARG, "retire needs integrated state");
+    const SceneGeom &g = ctx->scenes[s];
+    const size_t c = (size_t)g.p_count;
+    if (c) {
+        HIP_TRY(ctx, hipMemcpyAsync(pos4, ctx->pos[0] + g.p_begin, 16 * c, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(vel3, ctx->vel[0] + g.p_begin, 12 * c, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(ev4, ctx->ev[0] + g.p_begin, 16 * c, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(ctx, hipEventRecord(done, ctx->stream));
+    return GSPH_OK;
+}
+
+// Enqueues the copy of the device error flags into pinned host memory, then
+// clears them on the device (nothing else runs on the stream in between).
+int engine_flags_async(gsph_ctx *ctx, int *flags_pinned) {
+    launch_take_flags(ctx->dc, flags_pinned, ctx->stream);
+    return check_launch(ctx, "flags");
+}
+
+// One step of every resident scene; with `graph` the step is replayed from
+// a hipGraph captured on first use (dropped by the next re-layout).
+int engine_step(gsph_ctx *ctx, bool graph) {
+    if (!graph) return do_step(ctx);
+    if (!ctx->graph_exec) {
+        const int rc = capture_graph(ctx);
+        if (rc) return rc;
+    }
+    HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, ctx->stream));
+    ++ctx->steps;
+    ctx->cur = 0;
+    ctx->grid_valid = false;
+    ctx->masks_valid = false;
+    return GSPH_OK;
 }
 
 int engine_read_scene(gsph_ctx *ctx, int s, float *pos4, float *vel3, float *ev4) {
@@ -770,20 +988,6 @@ int engine_read_scene(gsph_ctx *ctx, int s, float *pos4, float *vel3, float *ev4
         if (ev4) HIP_TRY(ctx, hipMemcpyAsync(ev4, ctx->ev[b] + g.p_begin, 16 * c, hipMemcpyDeviceToHost, ctx->stream));
     }
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return GSPH_OK;
-}
-
-// Clears the sticky device error flags; returns them.
-int engine_take_errors(gsph_ctx *ctx, int *flags) {
-    int f = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(&f, ctx->dc + kCntErr, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    if (f) {
-        const int zero = 0;
-        HIP_TRY(ctx, hipMemcpyAsync(ctx->dc + kCntErr, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    }
-    *flags = f;
     return GSPH_OK;
 }
 
@@ -893,6 +1097,10 @@ int gsph_destroy(gsph_ctx *ctx) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     for (void *p : ctx->allocs) (void)hipFree(p);
+    for (auto &st : ctx->staging) {
+        if (st.done) (void)hipEventDestroy(st.done);
+        if (st.host) (void)hipHostFree(st.host);
+    }
     for (auto &e : ctx->stage_ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &p : ctx->kt.pending) {
@@ -952,6 +1160,7 @@ int grow_particles(gsph_ctx *ctx, int cap) {
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     const gsph_ctx saved = *ctx;   // host copy: pointers, sizes, bookkeeping
     ctx->allocs.clear();
+    ctx->alloc_bytes = 0;
     int rc = alloc_buffers(ctx, cap, saved.cap_c, saved.cap_s);
     const size_t n = (size_t)saved.n;
     hipStream_t s = ctx->stream;

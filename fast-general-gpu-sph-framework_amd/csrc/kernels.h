@@ -21,6 +21,14 @@ constexpr int kFillChunk = 2048;    // cells per fill block
 constexpr int kSplitTile = 1024;    // occupied cells per split tile
 constexpr int kMaskWords = 19;       // accepted-pair mask + row plan words per TRA particle
 
+// One scene's move in a serving re-layout: slots [src, src + count) of the
+// resident state go to [dst, dst + count), and its sorted keys shift by
+// key_delta (the change of the scene's first cell id, times 64).
+struct RelayoutSeg {
+    int src, dst, count;
+    int key_delta;
+};
+
 // Everything the arrange kernels touch.  Device pointers.
 struct ArrangeArgs {
     int n;                 // particles
@@ -132,6 +140,17 @@ void launch_split_count(const ArrangeArgs &a, bool batch, hipStream_t s);
 void launch_split_apply(const ArrangeArgs &a, bool batch, hipStream_t s);
 void launch_compact(const ArrangeArgs &a, hipStream_t s);
 void launch_empty_grid(const ArrangeArgs &a, hipStream_t s);
+// Serving re-layout: the first `total` slots of the destination state are the
+// continuing scenes' moved slots (segments sorted by dst, nseg of them),
+// keys shifted per segment; order flags set for the next arrange (the slot
+// order is the sorted order of the moved keys; new scenes carry keys that
+// never match, so they are sorted as movers).
+void launch_relayout(const float4 *pos_s, const f3 *vel_s, const float4 *ev_s,
+                     const uint32_t *hash_s, float4 *pos_d, f3 *vel_d, float4 *ev_d,
+                     uint32_t *hash_d, const RelayoutSeg *segs, int nseg, int total,
+                     hipStream_t s);
+void launch_set_order(int *dc, int order, int keys_ready, hipStream_t s);
+void launch_take_flags(int *dc, int *out_host, hipStream_t s);
 
 // physics.hip
 void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s);

@@ -2,16 +2,25 @@
 //
 // No counterpart in the reference (it runs one scene per process).  Requests
 // (a scene plus a step count) arrive over time.  Each engine iteration:
-//   1. admits arrived requests in arrival order while the HBM budget and the
-//      scene slots allow it (a request larger than the whole budget is
-//      rejected at submission);
-//   2. if the set of resident scenes changed, re-composes the batch context:
-//      continuing scenes keep their device state (device-to-device moves),
-//      new scenes are uploaded;
-//   3. advances every resident scene by one step with one launch per kernel;
-//   4. retires scenes whose step count is reached (state copied out) and
-//      fails scenes whose state left their box or stopped being finite,
-//      without disturbing the others.
+//   1. harvests finished requests whose state has reached host memory;
+//   2. admits arrived requests in arrival order while the HBM budget, the
+//      scene slots and the 2^26-cell grid limit allow it;
+//   3. if the set of resident scenes changed, re-lays out the batch context
+//      without a host synchronisation: continuing scenes keep their device
+//      state and their sorted order (one move kernel, keys shifted), new
+//      scenes are uploaded from pinned memory and appended;
+//   4. enqueues one step for every resident scene (one launch per kernel);
+//   5. scenes whose step count is reached are copied out asynchronously
+//      (pinned memory, an event per request) and leave the layout.
+// At most kInFlight steps are enqueued ahead of the GPU.  Device error flags
+// are copied to pinned memory after every step and read once that step has
+// completed: a scene whose state left its box or stopped being finite fails
+// alone (found by reading the resident states); a capacity or sort error ends
+// the run.  A finished scene's output is checked on the host before it is
+// reported done.
+// HBM accounting is the exact byte count of the engine's buffers at its
+// allocated capacity (engine_bytes), including both engines while a grown one
+// replaces the old.
 // Clock: realtime = 1 compares arrival times (s) with the wall clock;
 // realtime = 0 counts arrival in engine iterations (deterministic tests).
 #include <algorithm>
@@ -19,7 +28,6 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
-#include <map>
 #include <string>
 #include <thread>
 #include <vector>
@@ -33,20 +41,29 @@ using namespace gsph;
 namespace {
 
 enum ReqState { kQueued = 0, kActive = 1, kDone = 2, kFailed = 3, kRejected = 4 };
+constexpr int kInFlight = 3;                 // engine steps enqueued ahead of the GPU
+constexpr int kGraphAfter = 4;               // steps on one layout before it is captured
+constexpr long long kMaxCells = 1LL << 26;   // micro ids must fit 32 bits
+constexpr long long kMaxParticles = 0x7fffff00LL;
 
 struct Request {
-    int id;
-    float world[3];
-    int n;
-    std::vector<float> pos, vel;
-    int steps, left;
-    double arrival, admitted = -1, finished = -1;
+    int id = -1;
+    float world[3] = {0, 0, 0};
+    int n = 0;
+    int steps = 0, left = 0;
+    double arrival = 0, admitted = -1, finished = -1;
     int state = kQueued;
-    int slot = -1;         // scene index in the current layout
     long long cells = 0;
-    size_t bytes = 0;
-    std::vector<float> out_pos, out_vel, out_ev;
+    bool retiring = false;   // output copy in flight
+    float *in_pos = nullptr, *in_vel = nullptr;                        // pinned inputs
+    float *out_pos = nullptr, *out_vel = nullptr, *out_ev = nullptr;   // pinned outputs
+    hipEvent_t out_done = nullptr;
     std::string error;
+};
+
+struct StepRec {
+    hipEvent_t done = nullptr;
+    int *flags = nullptr;   // pinned
 };
 
 }  // namespace
@@ -59,11 +76,18 @@ struct gsph_server {
     gsph_ctx *ctx = nullptr;
     std::vector<Request> reqs;
     std::deque<int> queue;        // queued request ids in arrival order
-    std::vector<int> active;      // resident request ids in layout order
-    size_t used = 0, peak = 0;
-    long long engine_steps = 0, recompositions = 0, particle_steps = 0;
+    std::vector<int> layout;      // request ids of the engine's scenes, in layout order
+    std::vector<int> active;      // request ids that step: layout order, then newly admitted
+    std::vector<int> retiring;    // output copies in flight
+    long long act_n = 0, act_cells = 0;
+    size_t peak = 0;
+    long long engine_steps = 0, recompositions = 0, particle_steps = 0, grows = 0;
     int max_active = 0;
     double wall = 0;
+    std::vector<StepRec> ring;    // kInFlight + 1 step records
+    std::deque<int> pending;      // ring slots of enqueued, unchecked steps
+    int ring_next = 0;
+    long long since_layout = 0;   // steps enqueued on the current layout
     std::string err;
 };
 
@@ -75,74 +99,254 @@ int sfail(gsph_server *s, int code, const std::string &m) {
     return code;
 }
 
-long long cells_of(const gsph_constants &c, const float w[3]) {
-    long long t = 1;
-    for (int a = 0; a < 3; ++a) t *= (long long)std::ceil(w[a] / c.cell_size);
-    return t;
+bool scene_ok(const float *pos, int n, const float w[3]) {
+    for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            const float x = pos[4 * i + a];
+            if (!(x >= 0.f && x < w[a])) return false;
+        }
+        if (!std::isfinite(pos[4 * i + 3])) return false;
+    }
+    return true;
 }
 
-// Rebuilds the device layout for the current active set; grows the engine
-// when its capacity is exceeded.
-int recompose(gsph_server *s, const std::vector<int> &old_slots) {
+void free_request(Request &r) {
+    for (float **p : {&r.in_pos, &r.in_vel, &r.out_pos, &r.out_vel, &r.out_ev})
+        if (*p) {
+            (void)hipHostFree(*p);
+            *p = nullptr;
+        }
+    if (r.out_done) {
+        (void)hipEventDestroy(r.out_done);
+        r.out_done = nullptr;
+    }
+}
+
+size_t engine_now(const gsph_server *s) { return s->ctx ? engine_allocated(s->ctx) : 0; }
+
+void note_peak(gsph_server *s, size_t bytes) { s->peak = std::max(s->peak, bytes); }
+
+// Capacity the engine needs for n particles, c cells, k scenes: the current
+// one on every axis that suffices, else 1.5x growth (at least what is asked).
+void grown_caps(const gsph_server *s, long long n, long long c, long long k, long long caps[3]) {
+    int cn, cc, cs;
+    engine_capacity(s->ctx, &cn, &cc, &cs);
+    const long long cur[3] = {cn, cc, cs}, want[3] = {n, c, k};
+    for (int a = 0; a < 3; ++a)
+        caps[a] = want[a] <= cur[a] ? cur[a] : std::max(want[a], cur[a] * 3 / 2);
+    caps[0] = std::min(caps[0], kMaxParticles);
+    caps[1] = std::min(caps[1], kMaxCells);
+}
+
+bool fits_engine(const gsph_server *s, long long n, long long c, long long k) {
+    int cn, cc, cs;
+    engine_capacity(s->ctx, &cn, &cc, &cs);
+    return n <= cn && c <= cc && k <= cs;
+}
+
+// Device bytes held while an engine of capacity caps replaces the current
+// one: both are allocated during the move, unless no scene continues (then
+// the old engine is freed first).
+double growth_bytes(const gsph_server *s, const long long caps[3], bool continuing) {
+    return (double)engine_bytes(s->opt, caps[0], caps[1], caps[2]) +
+           (continuing ? (double)engine_now(s) : 0.0);
+}
+
+// Whether request r can join the active set now: grid limit, scene slots and
+// the HBM budget (with the transient double allocation of a growth).
+bool admissible(const gsph_server *s, const Request &r) {
+    const long long n = s->act_n + r.n, c = s->act_cells + r.cells;
+    const long long k = (long long)s->active.size() + 1;
+    if (c > kMaxCells || n > kMaxParticles || k > s->max_scenes) return false;
+    if (fits_engine(s, n, c, k)) return true;
+    const bool cont = !s->active.empty();
+    long long caps[3];
+    grown_caps(s, n, c, k, caps);
+    if (growth_bytes(s, caps, cont) <= s->budget) return true;
+    const long long exact[3] = {n, c, k};
+    return growth_bytes(s, exact, cont) <= s->budget;
+}
+
+// Re-lays out the engine for the active list: the continuing scenes in their
+// layout order, then the newly admitted ones.  Grows the engine (1.5x, or
+// exactly what is needed when the budget allows no more) when its capacity
+// is exceeded.
+int relayout(gsph_server *s) {
     std::vector<EngineScene> es;
-    long long n = 0, cells = 0;
-    for (size_t k = 0; k < s->active.size(); ++k) {
-        Request &r = s->reqs[s->active[k]];
+    std::vector<int> next;
+    std::vector<char> is_active(s->reqs.size(), 0);
+    for (int id : s->active) is_active[id] = 1;
+    std::vector<char> in_layout(s->reqs.size(), 0);
+    for (size_t k = 0; k < s->layout.size(); ++k) {
+        const int id = s->layout[k];
+        in_layout[id] = 1;
+        if (!is_active[id]) continue;
+        const Request &r = s->reqs[id];
         EngineScene e;
         std::memcpy(e.world, r.world, sizeof(e.world));
         e.count = r.n;
-        e.old_begin = old_slots[k];
-        e.pos4 = r.pos.data();
-        e.vel3 = r.vel.empty() ? nullptr : r.vel.data();
+        int first = 0, cb = 0;
+        engine_scene_layout(s->ctx, (int)k, &first, &cb);
+        e.old_begin = first;
+        e.old_cell_base = cb;
+        e.pos4 = nullptr;
+        e.vel3 = nullptr;
         es.push_back(e);
-        n += r.n;
-        cells += r.cells;
+        next.push_back(id);
     }
-    int cn, cc, cs;
-    engine_capacity(s->ctx, &cn, &cc, &cs);
+    const bool continuing = !es.empty();
+    for (int id : s->active) {
+        if (in_layout[id]) continue;
+        const Request &r = s->reqs[id];
+        EngineScene e;
+        std::memcpy(e.world, r.world, sizeof(e.world));
+        e.count = r.n;
+        e.old_begin = -1;
+        e.old_cell_base = 0;
+        e.pos4 = r.in_pos;
+        e.vel3 = r.in_vel;
+        es.push_back(e);
+        next.push_back(id);
+    }
+    const long long k = (long long)es.size();
     gsph_ctx *dst = s->ctx;
-    if (n > cn || cells > cc || (long long)es.size() > cs) {
-        const int nn = (int)std::max<long long>(n, (long long)cn * 3 / 2);
-        const int nc = (int)std::min<long long>(std::max<long long>(cells, (long long)cc * 3 / 2), 1LL << 26);
-        const int ns = std::max<int>((int)es.size(), cs * 3 / 2);
-        int rc = engine_create(&s->params, &s->opt, nn, nc, ns, &dst);
+    if (!fits_engine(s, s->act_n, s->act_cells, k)) {
+        long long caps[3];
+        grown_caps(s, s->act_n, s->act_cells, k, caps);
+        if (growth_bytes(s, caps, continuing) > s->budget) {
+            caps[0] = s->act_n;
+            caps[1] = s->act_cells;
+            caps[2] = k;
+        }
+        note_peak(s, (size_t)growth_bytes(s, caps, continuing));
+        if (!continuing) {   // nothing to keep: free the old engine first
+            gsph_destroy(s->ctx);
+            s->ctx = nullptr;
+        }
+        int rc = engine_create(&s->params, &s->opt, (int)caps[0], (int)caps[1], (int)caps[2], &dst);
         if (rc) return sfail(s, rc, std::string("engine grow: ") + gsph_last_error(nullptr));
+        ++s->grows;
     }
-    int rc = engine_recompose(s->ctx, dst, es);
+    int rc = engine_relayout(continuing ? s->ctx : nullptr, dst, es);
     if (rc) {
         const std::string m = gsph_last_error(dst);
         if (dst != s->ctx) gsph_destroy(dst);
-        return sfail(s, rc, "recompose: " + m);
+        return sfail(s, rc, "re-layout: " + m);
     }
     if (dst != s->ctx) {
-        gsph_destroy(s->ctx);
+        if (s->ctx) gsph_destroy(s->ctx);
         s->ctx = dst;
     }
-    for (size_t k = 0; k < s->active.size(); ++k) {
-        Request &r = s->reqs[s->active[k]];
-        r.slot = (int)k;
-        r.pos.clear();
-        r.pos.shrink_to_fit();
-        r.vel.clear();
-        r.vel.shrink_to_fit();
-    }
+    s->layout = next;
+    s->active = next;
+    s->since_layout = 0;
+    note_peak(s, engine_now(s));
     ++s->recompositions;
     return GSPH_OK;
 }
 
-bool scene_ok(const float *pos, int n, const gsph_constants &c, const float w[3]) {
-    for (int i = 0; i < n; ++i)
-        for (int a = 0; a < 3; ++a) {
-            const float x = pos[4 * i + a];
-            if (!(x >= 0.f && x < w[a]) || !std::isfinite(pos[4 * i + 3])) return false;
+// Reads every resident scene's state and fails those that left their box or
+// are not finite (after a device out-of-domain flag).  Synchronous; rare.
+int fail_broken_scenes(gsph_server *s, double now, bool &changed) {
+    std::vector<int> keep;
+    for (size_t k = 0; k < s->layout.size(); ++k) {
+        const int id = s->layout[k];
+        Request &r = s->reqs[id];
+        if (r.state != kActive || r.retiring) continue;   // a retiring output is checked on harvest
+        std::vector<float> pos(4 * (size_t)r.n);
+        int e = engine_read_scene(s->ctx, (int)k, pos.data(), nullptr, nullptr);
+        if (e) return sfail(s, e, gsph_last_error(s->ctx));
+        if (!scene_ok(pos.data(), r.n, r.world)) {
+            r.state = kFailed;
+            r.error = "state left the box or is not finite";
+            r.finished = now;
+            s->act_n -= r.n;
+            s->act_cells -= r.cells;
+            changed = true;
         }
-    (void)c;
-    return true;
+    }
+    for (int id : s->active)
+        if (s->reqs[id].state == kActive) keep.push_back(id);
+    s->active.swap(keep);
+    return GSPH_OK;
+}
+
+// Checks the error flags of the oldest enqueued step once it has completed
+// (blocking on it when `wait`).  Returns < 0 on a fatal device error.
+int check_oldest_step(gsph_server *s, bool wait, double now, bool &changed) {
+    if (s->pending.empty()) return GSPH_OK;
+    StepRec &sr = s->ring[s->pending.front()];
+    if (wait) {
+        if (hipEventSynchronize(sr.done) != hipSuccess)
+            return sfail(s, GSPH_ERR_HIP, "engine step failed");
+    } else if (hipEventQuery(sr.done) != hipSuccess) {
+        return GSPH_OK;
+    }
+    s->pending.pop_front();
+    const int flags = *sr.flags;
+    if (flags & (kErrCapacity | kErrSortStall))   // the whole batch is unusable
+        return sfail(s, (flags & kErrCapacity) ? GSPH_ERR_CAPACITY : GSPH_ERR_SPIN_TIMEOUT,
+                     "engine step: device error flags " + std::to_string(flags));
+    if (flags & kErrOutOfDomain) {   // find the scene(s)
+        const int rc = fail_broken_scenes(s, now, changed);
+        if (rc) return rc;
+    }
+    return GSPH_OK;
+}
+
+// Finishes requests whose output copy has completed.
+void harvest(gsph_server *s, bool wait_one, double now) {
+    std::vector<int> keep;
+    bool waited = false;
+    for (int id : s->retiring) {
+        Request &r = s->reqs[id];
+        bool done = hipEventQuery(r.out_done) == hipSuccess;
+        if (!done && wait_one && !waited) {
+            done = hipEventSynchronize(r.out_done) == hipSuccess;
+            waited = true;
+        }
+        if (!done) {
+            keep.push_back(id);
+            continue;
+        }
+        r.retiring = false;
+        r.finished = now;
+        if (scene_ok(r.out_pos, r.n, r.world)) {
+            r.state = kDone;
+        } else {
+            r.state = kFailed;
+            r.error = "state left the box or is not finite";
+        }
+        if (r.in_pos) {
+            (void)hipHostFree(r.in_pos);
+            r.in_pos = nullptr;
+        }
+        if (r.in_vel) {
+            (void)hipHostFree(r.in_vel);
+            r.in_vel = nullptr;
+        }
+    }
+    s->retiring.swap(keep);
 }
 
 }  // namespace
 
 extern "C" {
+
+int gsph_engine_bytes(const gsph_options *opt, long long particles, long long cells, int scenes,
+                      double *bytes) {
+    if (!bytes || particles < 0 || cells < 0 || scenes < 0)
+        return fail(nullptr, GSPH_ERR_INVALID_ARG, "bad capacity");
+    gsph_options o;
+    if (opt)
+        o = *opt;
+    else
+        gsph_options_default(&o);
+    if (o.sms_mask_bytes < 0 || o.sms_mask_bytes > 65536)
+        return fail(nullptr, GSPH_ERR_INVALID_ARG, "sms_mask_bytes must be in [0, 65536]");
+    *bytes = (double)engine_bytes(o, particles, cells, scenes);
+    return GSPH_OK;
+}
 
 int gsph_server_create(const gsph_params *p, const gsph_options *opt, double hbm_budget_bytes,
                        int max_scenes, gsph_server **out) {
@@ -156,17 +360,38 @@ int gsph_server_create(const gsph_params *p, const gsph_options *opt, double hbm
         gsph_options_default(&s->opt);
     s->budget = hbm_budget_bytes;
     s->max_scenes = max_scenes;
-    int rc = engine_create(p, &s->opt, 1 << 16, 1 << 16, 16, &s->ctx);
+    const int cap = 1 << 14;
+    if ((double)engine_bytes(s->opt, cap, cap, 4) > hbm_budget_bytes) {
+        delete s;
+        return sfail(nullptr, GSPH_ERR_BUDGET, "the budget does not hold the smallest engine");
+    }
+    int rc = engine_create(p, &s->opt, cap, cap, 4, &s->ctx);
     if (rc) {
         delete s;
         return rc;
     }
+    s->ring.resize(kInFlight + 1);
+    for (auto &sr : s->ring) {
+        if (hipEventCreateWithFlags(&sr.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void **)&sr.flags, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+            gsph_server_destroy(s);
+            return sfail(nullptr, GSPH_ERR_HIP, "server event / pinned allocation failed");
+        }
+        *sr.flags = 0;
+    }
+    note_peak(s, engine_now(s));
     *out = s;
     return GSPH_OK;
 }
 
 int gsph_server_destroy(gsph_server *s) {
     if (!s) return GSPH_OK;
+    if (s->ctx) (void)hipStreamSynchronize(engine_stream(s->ctx));
+    for (auto &r : s->reqs) free_request(r);
+    for (auto &sr : s->ring) {
+        if (sr.done) (void)hipEventDestroy(sr.done);
+        if (sr.flags) (void)hipHostFree(sr.flags);
+    }
     gsph_destroy(s->ctx);
     delete s;
     return GSPH_OK;
@@ -180,8 +405,6 @@ int gsph_server_submit(gsph_server *s, const float world3[3], int n, const float
     r.id = (int)s->reqs.size();
     std::memcpy(r.world, world3, sizeof(r.world));
     r.n = n;
-    r.pos.assign(pos4, pos4 + 4 * (size_t)n);
-    if (vel3) r.vel.assign(vel3, vel3 + 3 * (size_t)n);
     r.steps = r.left = steps;
     r.arrival = arrival;
     gsph_constants c;
@@ -189,21 +412,40 @@ int gsph_server_submit(gsph_server *s, const float world3[3], int n, const float
     std::memcpy(p.world_size, world3, sizeof(p.world_size));
     int rc = gsph_derive_constants(&p, &c);
     if (rc) return rc;
-    r.cells = cells_of(c, world3);
-    r.bytes = engine_bytes(n, r.cells);
-    if ((double)r.bytes > s->budget) {
+    r.cells = engine_cells(s->ctx, world3);
+    if (r.cells > kMaxCells) {
+        r.state = kRejected;
+        r.error = "more than 2^26 grid cells";
+    } else if ((double)engine_bytes(s->opt, n, r.cells, 1) > s->budget) {
         r.state = kRejected;
         r.error = "request exceeds the HBM budget";
-    } else if (!scene_ok(pos4, n, c, world3)) {
+    } else if (!scene_ok(pos4, n, world3)) {
         r.state = kRejected;
         r.error = "particles outside the box or not finite";
     }
-    if (r.state == kQueued && steps == 0) {   // nothing to simulate: the input is the result
-        r.state = kDone;
-        r.admitted = r.finished = arrival;
-        r.out_pos = r.pos;
-        r.out_vel = r.vel.empty() ? std::vector<float>(3 * (size_t)n, 0.f) : r.vel;
-        r.out_ev.assign(4 * (size_t)n, 0.f);
+    if (r.state == kQueued) {
+        const size_t nn = (size_t)std::max(n, 1);
+        if (hipHostMalloc((void **)&r.in_pos, 16 * nn, hipHostMallocDefault) != hipSuccess ||
+            (vel3 && hipHostMalloc((void **)&r.in_vel, 12 * nn, hipHostMallocDefault) != hipSuccess) ||
+            hipHostMalloc((void **)&r.out_pos, 16 * nn, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&r.out_vel, 12 * nn, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&r.out_ev, 16 * nn, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&r.out_done, hipEventDisableTiming) != hipSuccess) {
+            free_request(r);
+            return sfail(s, GSPH_ERR_HIP, "pinned host allocation for a request failed");
+        }
+        std::memcpy(r.in_pos, pos4, 16 * (size_t)n);
+        if (vel3) std::memcpy(r.in_vel, vel3, 12 * (size_t)n);
+        if (steps == 0) {   // nothing to simulate: the input is the result
+            r.state = kDone;
+            r.admitted = r.finished = arrival;
+            std::memcpy(r.out_pos, pos4, 16 * (size_t)n);
+            if (vel3)
+                std::memcpy(r.out_vel, vel3, 12 * (size_t)n);
+            else
+                std::memset(r.out_vel, 0, 12 * (size_t)n);
+            std::memset(r.out_ev, 0, 16 * (size_t)n);
+        }
     }
     *id = r.id;
     s->reqs.push_back(std::move(r));
@@ -223,93 +465,106 @@ int gsph_server_run(gsph_server *s, int realtime) {
         return (double)s->engine_steps;
     };
     bool changed = false;
-    std::vector<int> old_slots;
-    while (!s->queue.empty() || !s->active.empty()) {
-        const double now = clock();
-        // 1. admission, arrival order, budget and slots permitting
+    while (!s->queue.empty() || !s->active.empty() || !s->retiring.empty() ||
+           !s->pending.empty()) {
+        double now = clock();
+        // 1. completed steps' error flags, finished outputs
+        int rc = check_oldest_step(s, false, now, changed);
+        if (rc) return rc;
+        harvest(s, false, now);
+        // 2. admission, arrival order, budget / slots / grid permitting
         while (!s->queue.empty()) {
             Request &r = s->reqs[s->queue.front()];
-            if (r.arrival > now) break;
-            if ((int)s->active.size() >= s->max_scenes || (double)(s->used + r.bytes) > s->budget)
-                break;
+            if (r.arrival > now || !admissible(s, r)) break;
             s->queue.pop_front();
             r.state = kActive;
             r.admitted = now;
-            r.slot = -1;
-            s->used += r.bytes;
-            s->peak = std::max(s->peak, s->used);
             s->active.push_back(r.id);
+            s->act_n += r.n;
+            s->act_cells += r.cells;
             changed = true;
         }
         if (s->active.empty()) {
+            if (!s->pending.empty()) {
+                rc = check_oldest_step(s, true, clock(), changed);
+                if (rc) return rc;
+                continue;
+            }
+            if (!s->retiring.empty()) {
+                harvest(s, true, clock());
+                continue;
+            }
             if (s->queue.empty()) break;
             const double next = s->reqs[s->queue.front()].arrival;
-            if (realtime) {
+            if (next <= now) {
+                // alone, the head request always fits: its engine was checked
+                // against the budget at submission, and an engine with no
+                // continuing scene is freed before its replacement is made
+                Request &r = s->reqs[s->queue.front()];
+                s->queue.pop_front();
+                r.state = kActive;
+                r.admitted = now;
+                s->active.push_back(r.id);
+                s->act_n += r.n;
+                s->act_cells += r.cells;
+                changed = true;
+            } else if (realtime) {
                 const double wait = next - clock();
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+                continue;
             } else {
                 s->engine_steps = (long long)std::ceil(next);
+                continue;
             }
-            continue;
         }
-        // 2. re-compose
+        // 3. re-layout
         if (changed) {
-            old_slots.clear();
-            for (int id : s->active) {
-                const int slot = s->reqs[id].slot;
-                old_slots.push_back(slot < 0 ? -1 : engine_scene_begin(s->ctx, slot));
-            }
-            int rc = recompose(s, old_slots);
+            rc = relayout(s);
             if (rc) return rc;
             changed = false;
         }
         s->max_active = std::max(s->max_active, (int)s->active.size());
-        // 3. one step for every resident scene
-        int rc = gsph_step(s->ctx, 1, nullptr);
-        if (rc == GSPH_OK) {
-            rc = gsph_sync(s->ctx);
-            if (rc == GSPH_ERR_OUT_OF_DOMAIN) rc = 1;   // find the scene(s) below
+        // 4. one step for every resident scene, at most kInFlight ahead
+        while ((int)s->pending.size() >= kInFlight) {
+            rc = check_oldest_step(s, true, clock(), changed);
+            if (rc) return rc;
         }
-        if (rc < 0) return sfail(s, rc, std::string("engine step: ") + gsph_last_error(s->ctx));
+        if (changed) continue;   // a scene failed: lay out again before stepping
+        // a layout that has lasted a few steps is replayed from a graph
+        // (one launch instead of one per kernel); a re-layout drops it
+        rc = engine_step(s->ctx, s->since_layout >= kGraphAfter);
+        ++s->since_layout;
+        if (rc) return sfail(s, rc, std::string("engine step: ") + gsph_last_error(s->ctx));
+        StepRec &sr = s->ring[s->ring_next];
+        rc = engine_flags_async(s->ctx, sr.flags);
+        if (!rc && hipEventRecord(sr.done, engine_stream(s->ctx)) != hipSuccess) rc = GSPH_ERR_HIP;
+        if (rc) return sfail(s, rc, "engine step bookkeeping failed");
+        s->pending.push_back(s->ring_next);
+        s->ring_next = (s->ring_next + 1) % (int)s->ring.size();
         ++s->engine_steps;
-        // 4. retire finished scenes, fail broken ones
+        // 5. retire finished scenes
         std::vector<int> keep;
-        for (int id : s->active) {
+        for (size_t k = 0; k < s->layout.size(); ++k) {
+            const int id = s->layout[k];
             Request &r = s->reqs[id];
+            if (r.state != kActive) continue;
             --r.left;
             s->particle_steps += r.n;
-            bool broken = false;
-            if (rc == 1 || r.left == 0) {
-                std::vector<float> pos(4 * (size_t)r.n), vel(3 * (size_t)r.n), ev(4 * (size_t)r.n);
-                int e = engine_read_scene(s->ctx, r.slot, pos.data(), vel.data(), ev.data());
-                if (e) return sfail(s, e, gsph_last_error(s->ctx));
-                if (rc == 1) broken = !scene_ok(pos.data(), r.n, gsph_constants(), r.world);
-                if (!broken && r.left == 0) {
-                    r.out_pos.swap(pos);
-                    r.out_vel.swap(vel);
-                    r.out_ev.swap(ev);
-                }
-            }
-            if (broken) {
-                r.state = kFailed;
-                r.error = "state left the box or is not finite";
-            } else if (r.left == 0) {
-                r.state = kDone;
-            }
-            if (r.state == kDone || r.state == kFailed) {
-                r.finished = clock();
-                s->used -= r.bytes;
-                changed = true;
-            } else {
+            if (r.left > 0) {
                 keep.push_back(id);
+                continue;
             }
-        }
-        if (rc == 1) {
-            int flags = 0;
-            engine_take_errors(s->ctx, &flags);
+            rc = engine_retire_async(s->ctx, (int)k, r.out_pos, r.out_vel, r.out_ev, r.out_done);
+            if (rc) return sfail(s, rc, gsph_last_error(s->ctx));
+            r.retiring = true;
+            s->retiring.push_back(id);
+            s->act_n -= r.n;
+            s->act_cells -= r.cells;
+            changed = true;
         }
         s->active.swap(keep);
     }
+    if (s->ctx) (void)hipStreamSynchronize(engine_stream(s->ctx));
     s->wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return GSPH_OK;
 }
@@ -324,9 +579,9 @@ int gsph_server_result(gsph_server *s, int id, float *pos4, float *vel3, float *
     if (queued) *queued = r.admitted >= 0 ? r.admitted - r.arrival : -1;
     if (r.state == kDone) {
         const size_t n = (size_t)r.n;
-        if (pos4) std::memcpy(pos4, r.out_pos.data(), 16 * n);
-        if (vel3) std::memcpy(vel3, r.out_vel.data(), 12 * n);
-        if (ev4) std::memcpy(ev4, r.out_ev.data(), 16 * n);
+        if (pos4) std::memcpy(pos4, r.out_pos, 16 * n);
+        if (vel3) std::memcpy(vel3, r.out_vel, 12 * n);
+        if (ev4) std::memcpy(ev4, r.out_ev, 16 * n);
     }
     return GSPH_OK;
 }

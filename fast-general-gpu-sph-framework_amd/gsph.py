@@ -158,6 +158,8 @@ _SIGS = {
     "gsph_kernel_timing": (_I, [_P, _I]),
     "gsph_kernel_times": (_I, [_P, C.c_char_p, _I, _P, _P, _I, C.POINTER(_I)]),
     "gsph_inject_fault": (_I, [_P, _I]),
+    "gsph_engine_bytes": (_I, [C.POINTER(Options), C.c_longlong, C.c_longlong, _I,
+                               C.POINTER(C.c_double)]),
     "gsph_server_create": (_I, [C.POINTER(Params), C.POINTER(Options), C.c_double, _I,
                                 C.POINTER(_P)]),
     "gsph_server_destroy": (_I, [_P]),
@@ -273,6 +275,14 @@ def probe_valu(iters: int = 4000):
     return ms.value, ops.value
 
 
+def engine_bytes(particles: int, cells: int, scenes: int = 1, options: Options | None = None) -> float:
+    """Device bytes of a context with these capacities (host only)."""
+    b = C.c_double()
+    _check(load().gsph_engine_bytes(None if options is None else C.byref(options), int(particles),
+                                    int(cells), int(scenes), C.byref(b)))
+    return b.value
+
+
 def scene_config(k: int):
     """Synthetic workload k (0, 1, 2) -> (Params, pos4 array)."""
     lib = load()
@@ -316,6 +326,24 @@ def batch_workload(count=256, seed=20261206):
         blocks.append(scene_block([0.06, 0.06, 0.06], nx, ny, nz, 0.01, 0.01, seed + 7919 * (k + 1)))
         worlds.append([2 * nx * 0.01, 2 * ny * 0.01, 2 * nz * 0.01])
     return worlds, blocks
+
+
+def serving_workload(count=96, seed=20261207):
+    """The serving workload's requests: N log-uniform in [8192, 1048576] (a
+    cube lattice block of side round(N^(1/3)), spacing 0.01, jitter 1%,
+    corner 0.06, in a cubic box 2x its extent) and a step count uniform in
+    50..500.  Returns (worlds, blocks, steps, rng); the caller draws the
+    arrival times from rng (Poisson at its chosen rate)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(8192), np.log(1048576), count))
+    steps = rng.integers(50, 501, count)
+    worlds, blocks = [], []
+    for k in range(count):
+        side = max(8, int(round(ns[k] ** (1 / 3))))
+        blocks.append(scene_block([0.06, 0.06, 0.06], side, side, side, 0.01, 0.01, seed + k))
+        w = 2 * side * 0.01
+        worlds.append([w, w, w])
+    return worlds, blocks, [int(x) for x in steps], rng
 
 
 class System:

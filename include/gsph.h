@@ -204,12 +204,18 @@ int gsph_snapshot_load(const char *path, const gsph_options *opt, gsph_ctx **out
 /* Serving form --------------------------------------------------------------- */
 /* No counterpart in the reference.  A continuous-batching request loop:
  * requests (one scene + a step count) are admitted in arrival order against
- * an HBM budget and a scene-slot limit, all resident scenes advance together
- * one step per engine iteration, finished scenes are copied out and their
- * memory reused.  A request larger than the budget, or with particles outside
- * its box, is rejected at submission; a scene whose state stops being finite
- * fails alone.  run(realtime = 1) takes arrival times in seconds against the
- * wall clock; realtime = 0 counts them in engine iterations. */
+ * an HBM budget (exact engine bytes, gsph_engine_bytes), a scene-slot limit
+ * and the 2^26-cell grid limit (a request that does not fit yet waits); all
+ * resident scenes advance together one step per engine iteration with no host
+ * synchronisation per step (at most 3 steps in flight, device error flags
+ * read one step behind); the layout changes only when scenes join or leave,
+ * and then without a host synchronisation (continuing scenes keep their
+ * device state and sorted order); finished scenes are copied out
+ * asynchronously into pinned host memory.  A request larger than the budget,
+ * or with particles outside its box, is rejected at submission; a scene whose
+ * state leaves its box or stops being finite fails alone.  run(realtime = 1)
+ * takes arrival times in seconds against the wall clock; realtime = 0 counts
+ * them in engine iterations. */
 typedef struct gsph_server gsph_server;
 typedef struct gsph_server_stats {
     int completed, failed, rejected, max_active;
@@ -224,6 +230,14 @@ typedef struct gsph_server_stats {
 #define GSPH_REQ_DONE 2
 #define GSPH_REQ_FAILED 3
 #define GSPH_REQ_REJECTED 4
+/* Device bytes of a context whose buffers hold `particles` particles, `cells`
+ * grid cells and `scenes` scenes under options opt (NULL: defaults): exactly
+ * what the library allocates for those capacities (every context checks its
+ * allocation against it).  The serving loop charges its engine by this,
+ * including both engines while a grown one replaces the old; a request is
+ * rejected when an engine sized for it alone exceeds the budget.  Host only. */
+int gsph_engine_bytes(const gsph_options *opt, long long particles, long long cells, int scenes,
+                      double *bytes);
 int gsph_server_create(const gsph_params *p, const gsph_options *opt,
                        double hbm_budget_bytes, int max_scenes, gsph_server **out);
 int gsph_server_destroy(gsph_server *s);

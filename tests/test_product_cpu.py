@@ -166,3 +166,34 @@ def test_snapshot_check_validates_without_gpu(gsph, tmp_path):
         with pytest.raises(gsph.GsphError) as e:
             gsph.snapshot_check(bad)
         assert e.value.code == -7, name
+
+
+def test_engine_bytes_accounting(gsph):
+    """gsph_engine_bytes, the serving loop's HBM accounting (every context
+    checks its own allocation against it on creation): the SMS mask pool
+    costs exactly sms_mask_bytes per particle of capacity (the largest
+    setting, 65536, included), each further particle, cell or scene costs a
+    fixed amount, and the totals grow with every capacity."""
+    o = gsph.options_default()
+    base = dict(particles=1 << 17, cells=1 << 20, scenes=8)
+
+    def b(sms, **kw):
+        o.sms_mask_bytes = sms
+        a = dict(base, **kw)
+        return gsph.engine_bytes(a["particles"], a["cells"], a["scenes"], o)
+
+    n = base["particles"]
+    assert b(65536) - b(256) == n * (65536 - 256)
+    assert b(1024) - b(256) == n * (1024 - 256)
+    # the largest setting dominates: more than 64 KiB per particle in all
+    assert b(65536) > n * 65536
+    # per particle (state twice, sort lists, masks and plans, tasks, ...):
+    # at least the 88 bytes of the two state copies
+    per_particle = (b(0, particles=n + 4096) - b(0)) / 4096
+    assert per_particle >= 88
+    # per cell: begin, req, TRA/SMS bases (4 B each), x-layer starts (8 B)
+    per_cell = (b(0, cells=base["cells"] + (1 << 16)) - b(0)) / (1 << 16)
+    assert 24 <= per_cell < 25
+    assert b(0, scenes=9) > b(0)
+    with pytest.raises(gsph.GsphError):
+        b(65537)
