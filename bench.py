@@ -1,15 +1,17 @@
 """Benchmark of the SPH step on one MI355X.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cpu-steps S]
-                [--legs plain,batched,serving,reference] [--no-autotune]
+                [--legs plain,batched,serving,reference,skewed] [--no-autotune]
 
 Workload: the 158^3 = 3,944,312-particle column in a 6.32 m cube (317^3
-cells), h = 0.02, dt = 1e-4, FP32 (synthetic workload 1).  A step is one
-pass of the hot path (arrange, density, force, integrate) over that state,
-resident in HBM.  Prints one JSON line; beside the plain step it carries the
-batched (256 scenes) and serving (Poisson requests) legs, the reference's
-own default scene timed against its published frame time, and the CPU
-baseline.
+cells), h = 0.02, dt = 1e-4, FP32 (synthetic workload 1), 200 timed steps by
+default.  A step is one pass of the hot path (arrange, density, force,
+integrate) over that state, resident in HBM.  Prints one JSON line; beside
+the plain step it carries the batched (256 scenes), serving (Poisson
+requests), skewed-occupancy (pool + fine sphere, 8.1 M particles) legs, the
+reference's own default scene timed against its published frame time, and
+the CPU baseline with a per-quantity parity check of workload 0 against the
+FP64 oracle.
 """
 import argparse
 import json
@@ -51,22 +53,45 @@ def algorithmic_bytes(n, ncell, fused=False):
     }
 
 
-def cpu_baseline(params, pos, steps, threads):
+def cpu_baseline(gsph, params, pos, steps, threads):
+    """The CPU baseline (the oracle's FP32 OpenMP restatement of the reference
+    step, timed on `threads` host threads over `steps` steps of the workload)
+    and, with the oracle as the checker, the per-quantity errors of one GPU
+    step of workload 0 against its FP64 step."""
+    import numpy as np
     sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
     import pyoracle as po
+    from tolerance import step_errors
 
     op = po.params_from(params.as_dict())
     t0 = time.perf_counter()
     po.step_f32(op, pos, nsteps=steps, nthreads=threads)
     dt = time.perf_counter() - t0
-    return {
+    base = {
         "value": pos.shape[0] * steps / dt,
         "unit": "particle-steps/s",
         "cores": threads,
+        "nproc": os.cpu_count(),
         "kind": "port",
-        "sample": f"FP32 C++ restatement of the reference step (oracle/oracle.cpp, OpenMP), "
-                  f"{steps} steps of the same 3,944,312-particle workload, {dt:.1f} s",
+        "sample": f"FP32 C++ restatement of the reference step (oracle/oracle.cpp, OpenMP on "
+                  f"{threads} threads, the job's CPU share), {steps} steps of the same "
+                  f"3,944,312-particle workload, {dt:.1f} s",
     }
+    p0, pos0 = gsph.scene_config(0)
+    n0 = pos0.shape[0]
+    s = gsph.System(p0, pos0)
+    s.step(1)
+    got = s.read_state()
+    s.close()
+    op0 = po.params_from(p0.as_dict())
+    ref = po.fp64_step(op0, pos0, np.zeros((n0, 3), np.float32), np.zeros((n0, 4), np.float32))
+    errs = step_errors(*got, ref, op0)
+    parity = {"workload": f"workload 0 (32^3 = {n0} particles), one step from rest",
+              "bar": 1e-5, "max_error": errs,
+              "measure": "each quantity relative to the scale its FP32 rounding follows "
+                         "(tests/tolerance.py)"}
+    return base, parity
 
 
 def opt_dict(o):
@@ -151,13 +176,49 @@ def reference_scene_leg(gsph, steps=20, warmup=3, no_autotune=False):
     n = pos.shape[0]
     return {
         "workload": "the reference's default scene (assets/scene_default.json): 3,944,312 "
-                    "particles, h=0.03, ~296 per cell (SMS path)",
+                    "particles, h=0.03, ~296 per cell",
         "ms_per_step": ms, "particle_steps_per_s": n * 1e3 / ms,
         "stages_ms": {k: sum(x[k] for x in st) / len(st) for k in st[0]},
-        "sms_tasks": info["tasks"],
+        "sms_tasks": info["tasks"], "tra_particles": info["tra"],
+        "path": ("SMS tasks for every dense cell" if info["tra"] == 0 else
+                 "TRA for every cell (long rows keep their pair masks in the pool)"
+                 if info["tasks"] == 0 else "mixed TRA / SMS"),
+        "path_reason": "the autotuner times the reference split rule, tuned thresholds and "
+                       "all-TRA and keeps the fastest",
         "reference_published_ms": REFERENCE_SCENE_MS,
         "reference_published_hw": "RTX 5090 (README.md:212)",
         "speedup_vs_reference_published": REFERENCE_SCENE_MS / ms,
+        "autotuned_options": opt_dict(tuned),
+    }
+
+
+def skewed_leg(gsph, steps=100, warmup=5, no_autotune=False):
+    """Workload 2: a 0.01-spaced pool (256 x 64 x 256) plus a 0.005-spaced
+    sphere dropped from 1 m above (8.1 M particles, 0 to 64 per cell): the
+    configuration that exercises the TRA/SMS split.  Autotuned including the
+    split; per-stage times and the chosen split."""
+    params, pos = gsph.scene_config(2)
+    s = gsph.System(params, pos)
+    tuned, tuned_ms = (s.options(), None) if no_autotune else s.autotune(trial_steps=2,
+                                                                           tune_split=True)
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    st = s.step(10, timings=True)
+    info = s.grid_info()
+    s.close()
+    n = pos.shape[0]
+    return {
+        "workload": f"pool 256x64x256 at 0.01 + sphere at 0.005 ({n} particles), h=0.02, "
+                    f"{steps} timed steps",
+        "particles": n, "ms_per_step": ms, "particle_steps_per_s": n * 1e3 / ms,
+        "stages_ms": {k: sum(x[k] for x in st) / len(st) for k in st[0]},
+        "split": {"split_mode": tuned.split_mode, "sms_min_self": tuned.sms_min_self,
+                  "sms_min_total": tuned.sms_min_total, "tra_particles": info["tra"],
+                  "sms_tasks": info["tasks"]},
         "autotuned_options": opt_dict(tuned),
     }
 
@@ -223,12 +284,13 @@ def serving_leg(gsph, requests=96, seed=20261207, budget=200e9, load=0.7):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--legs", default="plain,batched,serving,reference")
+    ap.add_argument("--legs", default="plain,batched,serving,reference,skewed")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default options (profiling runs: no trial launches)")
     ap.add_argument("--probe-copy", type=float, default=0.0,
@@ -343,7 +405,13 @@ def main():
         "config": {"workload": "plain single scene: 158^3 = 3,944,312-particle column in a "
                                "6.32 m cube (317^3 cells), h=0.02, dt=1e-4, FP32",
                    "particles": n, "cells": ncell, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+        # "bound" is the roofline this line is priced against (HBM: the
+        # path has no dense contraction for MFMA); "limiter" is what the
+        # counters show actually bounds the kernel, the larger fraction
+        "roofline": {"bound": "hbm", "limiter": ("valu" if valu_achieved and
+                                                 valu_achieved / valu_peak > achieved / HBM_PEAK_GBS
+                                                 else "hbm"),
+                     "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": dom_traffic, "algorithmic_bytes": alg[dom], "avg_ms": dom_ms,
                      "traffic_source": TRAFFIC_PROFILE.name if dom_traffic else None,
@@ -365,8 +433,7 @@ def main():
     legs = args.legs.split(",")
     if "batched" in legs:
         print("[bench] batched leg", file=sys.stderr, flush=True)
-        out["batched"] = batched_leg(gsph, max(args.steps // 2, 5), 3,
-                                     no_autotune=args.no_autotune)
+        out["batched"] = batched_leg(gsph, args.batched_steps, 3, no_autotune=args.no_autotune)
         print(f"[bench] batched {out['batched']['speedup_vs_plain_loop']:.2f}x plain loop",
               file=sys.stderr, flush=True)
     if "serving" in legs:
@@ -380,13 +447,22 @@ def main():
         out["reference_scene"] = reference_scene_leg(gsph, no_autotune=args.no_autotune)
         print(f"[bench] reference scene {out['reference_scene']['ms_per_step']:.2f} ms/step",
               file=sys.stderr, flush=True)
+    if "skewed" in legs:
+        print("[bench] skewed-occupancy leg", file=sys.stderr, flush=True)
+        out["skewed"] = skewed_leg(gsph, no_autotune=args.no_autotune)
+        print(f"[bench] skewed {out['skewed']['ms_per_step']:.2f} ms/step", file=sys.stderr,
+              flush=True)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        # the job's CPU share (OMP_NUM_THREADS is set to it on the GPU box;
+        # nproc may show the whole host)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         print(f"[bench] cpu baseline {args.cpu_steps} steps on {threads} threads",
               file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(params, pos, args.cpu_steps, threads)
+        out["cpu_baseline"], out["parity"] = cpu_baseline(gsph, params, pos, args.cpu_steps,
+                                                          threads)
     else:
         out["cpu_baseline"] = None
+        out["parity"] = None
     print(json.dumps(out), flush=True)
 
 

@@ -253,6 +253,7 @@ __device__ __forceinline__ bool in_range(float d2, PairRange pr) {
 struct SmsItem {
     bool has_task, active, shared;
     int self, si, gidx, cx, cy, cz, gl, gr;
+    int cell_count;   // particles of the task's cell
 };
 
 template <bool kBatch>
@@ -267,6 +268,7 @@ __device__ __forceinline__ SmsItem sms_item(const PhysicsArgs &P, int t0, int T,
     m.active = m.has_task && m.self < cell_end;
     m.si = m.has_task ? min(m.self, cell_end - 1) : 0;
     m.gidx = m.has_task ? scene_of_particle<kBatch>(P.scenes, P.nscenes, tk.cell_begin) : 0;
+    m.cell_count = m.has_task ? tk.cell_particle_count : 0;
     m.cx = m.has_task ? tk.cell_pos[0] : 0;
     m.cy = m.has_task ? tk.cell_pos[1] : 0;
     m.cz = m.has_task ? tk.cell_pos[2] : 0;
@@ -309,10 +311,16 @@ __device__ __forceinline__ void density_store(const PhysicsArgs &P, int i, float
 // The rows of one TRA lane: live flag and sorted range per row.
 template <bool kBatch>
 __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi, bool active,
-                                         int i, int lo[9], int hi[9]) {
+                                         int i, int lo[9], int hi[9], int *p_begin = nullptr,
+                                         int *cell_count = nullptr) {
     const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+    if (p_begin) *p_begin = g.p_begin;
     int mx, my, mz;
     micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
+    if (cell_count) {
+        const int c = g.cell_base + (mx >> 2) + g.gx * ((my >> 2) + g.gy * (mz >> 2));
+        *cell_count = P.begin[c + 1] - P.begin[c];
+    }
     RowPlan rp;
     plan_rows(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, P.row_prune != 0, rp);
 #pragma unroll
@@ -449,17 +457,84 @@ struct DensLds {
     static constexpr int kWords = 2 * 9 * 64 / 4 + kStage;    // float4s: rows + round
 };
 
-// One row of a TRA lane from staged candidates q[0, cnt): per-row sum rs (a
-// rejected candidate adds an exact zero) and accept mask mk (bits in
-// candidate order enter at the bottom).
-__device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, int cnt, float h2,
-                                            PairRange pr, float &rs, uint32_t &mk) {
-    int k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs, mk);
+// Per-row density sums.  A particle in a cell of at most kFourChainCell
+// particles sums each row in one sequential chain (the reference's order).
+// In denser cells a row holds ~100+ candidates (~130 at the reference
+// scene's density), and one chain lets the FP32 rounding error grow with that
+// length, which the 7th power of the equation of state amplifies 7x in the
+// pressure; there the row is summed in four interleaved chains: the candidate
+// with place j in its scene (sorted index minus the scene's first) goes to
+// chain j & 3, and the row's sum is (a0 + a1) + (a2 + a3).  Four chains a
+// quarter as long keep density and pressure within 1e-5 of FP64 at the
+// reference scene's density.  Both the choice (the particle's own cell
+// count) and the chain of a candidate depend on the particle and the
+// candidate alone, and rejected candidates add exact zeros, so every path
+// (TRA windows of any depth, pruned or not, SMS batches, pool sub-windows) and
+// every place of the scene in a batch gets the same bits.
+constexpr int kFourChainCell = 48;
+
+struct RowSum {
+    float s1;                  // one chain
+    float a0, a1, a2, a3;      // four chains
+};
+
+__device__ __forceinline__ RowSum row_sum_zero() { return {0.f, 0.f, 0.f, 0.f, 0.f}; }
+
+__device__ __forceinline__ float row_sum_value(const RowSum &r, bool four) {
+    return four ? (r.a0 + r.a1) + (r.a2 + r.a3) : r.s1;
+}
+
+// (b0..b3) = (a_q, a_{q+1}, a_{q+2}, a_{q+3}) indices mod 4 (q = 0..3, per
+// lane): two conditional swaps
+__device__ __forceinline__ void chains_rotate(float &a0, float &a1, float &a2, float &a3, int q) {
+    if (q & 2) {
+        const float t0 = a0, t1 = a1;
+        a0 = a2;
+        a1 = a3;
+        a2 = t0;
+        a3 = t1;
     }
-    for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs, mk);
+    if (q & 1) {
+        const float t = a0;
+        a0 = a1;
+        a1 = a2;
+        a2 = a3;
+        a3 = t;
+    }
+}
+
+// Candidates q[0, cnt) of a row, the first at place off of its scene; accept
+// mask mk (bits in candidate order enter at the bottom).
+__device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, int cnt, int off,
+                                            bool four, float h2, PairRange pr, RowSum &rs,
+                                            uint32_t &mk) {
+    if (!four) {
+        int k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs.s1, mk);
+        }
+        for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs.s1, mk);
+    } else {
+        const int r = off & 3;
+        float a0 = rs.a0, a1 = rs.a1, a2 = rs.a2, a3 = rs.a3;
+        chains_rotate(a0, a1, a2, a3, r);
+        int k = 0;
+        for (; k + 4 <= cnt; k += 4) {
+            density_pair(pi, q[k], h2, pr, a0, mk);
+            density_pair(pi, q[k + 1], h2, pr, a1, mk);
+            density_pair(pi, q[k + 2], h2, pr, a2, mk);
+            density_pair(pi, q[k + 3], h2, pr, a3, mk);
+        }
+        if (k < cnt) density_pair(pi, q[k], h2, pr, a0, mk);
+        if (k + 1 < cnt) density_pair(pi, q[k + 1], h2, pr, a1, mk);
+        if (k + 2 < cnt) density_pair(pi, q[k + 2], h2, pr, a2, mk);
+        chains_rotate(a0, a1, a2, a3, (4 - r) & 3);
+        rs.a0 = a0;
+        rs.a1 = a1;
+        rs.a2 = a2;
+        rs.a3 = a3;
+    }
 }
 
 template <bool kBatch, int kStage>
@@ -486,8 +561,9 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             const bool active = idx < tra;
             const int i = P.compact[active ? idx : tra - 1];
             const float4 pi = P.pos[i];
-            int lo[9], hi[9];
-            tra_rows<kBatch>(P, pi, active, i, lo, hi);
+            int lo[9], hi[9], pb = 0, own = 0;
+            tra_rows<kBatch>(P, pi, active, i, lo, hi, &pb, &own);
+            const bool four = own > kFourChainCell;
             bool lng = false;
 #pragma unroll
             for (int r = 0; r < 9; ++r) lng |= active && hi[r] - lo[r] > kMaskRowMax;
@@ -515,13 +591,13 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                 int round = 0;
                 // per-row partial sums (rejected candidates add exact zeros, so
                 // pruning cannot change the result) and per-row accept masks
-                float rs = 0.f;
+                RowSum rs = row_sum_zero();
                 uint32_t mk = 0;
                 int done = 0;   // rows at order positions [0, done) are finished
                 auto finish_rows = [&](int upto) __attribute__((always_inline)) {
                     for (; done < upto; ++done) {
-                        s += rs;
-                        rs = 0.f;
+                        s += row_sum_value(rs, four);
+                        rs = row_sum_zero();
                         if (P.use_masks && !pool) {
                             const int r = row_of(done);
                             const int cnt = rhi[r * 64 + lane] - rlo[r * 64 + lane];
@@ -555,7 +631,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                             const int nc = ce - cb;
                             uint32_t w = 0;
                             if (nc > 0) {
-                                density_row(pi, stg + (cb - base), nc, h2, pr, rs, w);
+                                density_row(pi, stg + (cb - base), nc, cb - pb, four, h2, pr, rs, w);
                                 w = finish_mask(w, nc) << (cb - base - 32 * c);
                             }
                             if (round < pool_rounds)   // always, by the bound
@@ -563,7 +639,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                         }
                         ++round;
                     } else if (cnt > 0) {
-                        density_row(pi, stg + (jb - base), cnt, h2, pr, rs, mk);
+                        density_row(pi, stg + (jb - base), cnt, jb - pb, four, h2, pr, rs, mk);
                     }
                     wave_sync_lds();
                 }
@@ -590,6 +666,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             const SmsItem sm = sms_item<kBatch>(P, item << 1, T, lane);
             const float4 pi = P.pos[sm.si];
             const SceneGeom g = P.scenes[sm.gidx];
+            const bool four = sm.cell_count > kFourChainCell;
             const int width = sm.shared ? 64 : 32;
             const int sl = sm.shared ? lane : (lane & 31);
             float4 *lds = sm.shared ? wl : wl + (lane & 32);
@@ -628,7 +705,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 const int r = row_of(q);
-                float rs = 0.f;
+                RowSum rs = row_sum_zero();
                 // the next batch's candidate is loaded while this one computes
                 float4 nx = pi;
                 if (sl < min(width, hi[r] - lo[r])) nx = P.pos[lo[r] + sl];
@@ -640,12 +717,12 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                     for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                         const int n = min(32, cnt - k0);
                         uint32_t mk = 0;
-                        density_row(pi, lds + k0, n, h2, pr, rs, mk);
+                        density_row(pi, lds + k0, n, b0 + k0 - g.p_begin, four, h2, pr, rs, mk);
                         if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_mask(mk, n);
                     }
                     wave_sync_lds();
                 }
-                s += rs;
+                s += row_sum_value(rs, four);
             }
             if (sm.active) density_store(P, sm.self, s);
         }
