@@ -1591,6 +1591,31 @@ int gsph_checksum_range(gsph_ctx *ctx, int first, int count, double out[5]) {
     return collect_errors(ctx);
 }
 
+int gsph_checksum_ranges_noalloc(gsph_ctx *ctx, int nranges, const int *first, const int *count,
+                                 double *out5_device) {
+    if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
+    if (nranges < 0 || (nranges > 0 && (!first || !count || !out5_device)))
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "bad ranges or output");
+    if (nranges > kNoAllocRanges)
+        return fail(ctx, GSPH_ERR_CAPACITY,
+                    "more than " + std::to_string(kNoAllocRanges) +
+                        " ranges per call (the size of the static scratch)");
+    for (int k = 0; k < nranges; ++k)
+        if (first[k] < 0 || count[k] < 0 || (long long)first[k] + count[k] > ctx->n)
+            return fail(ctx, GSPH_ERR_INVALID_ARG, "bad checksum range " + std::to_string(k));
+    int dev = -1;
+    HIP_TRY(ctx, hipGetDevice(&dev));
+    if (dev != ctx->device)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "the context's device is not the current device");
+    double *scr = nullptr;
+    HIP_TRY(ctx, checksum_scratch(&scr));
+    for (int k = 0; k < nranges; ++k)
+        launch_checksum(ctx->pos[ctx->cur] + first[k], count[k],
+                        scr + (size_t)k * kChecksumBlocks * 5, kChecksumBlocks, out5_device + 5 * k,
+                        ctx->stream);
+    return check_launch(ctx, "checksum");
+}
+
 int gsph_checksum(gsph_ctx *ctx, double out[5]) {
     if (!ctx) return fail(nullptr, GSPH_ERR_INVALID_ARG, "null ctx");
     return gsph_checksum_range(ctx, 0, ctx->n, out);

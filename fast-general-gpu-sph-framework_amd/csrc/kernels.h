@@ -156,6 +156,12 @@ void launch_take_flags(int *dc, int *out_host, hipStream_t s);
 void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s);
+constexpr int kChecksumBlocks = 256;   // blocks of the FP64 checksum's first pass
+constexpr int kNoAllocRanges = 64;     // ranges per call of the no-allocation checksum
+constexpr int kMaxDevices = 64;
+// Device address (current device) of the no-allocation checksum's static
+// scratch: kNoAllocRanges * kChecksumBlocks * 5 doubles.
+hipError_t checksum_scratch(double **ptr);
 void launch_checksum(const float4 *pos, int n, double *partial, int nblocks,
                      double *out5, hipStream_t s);
 void launch_init_state(float4 *ev, f3 *acc, int n, hipStream_t s);

@@ -23,6 +23,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <mutex>
 
 // No implicit fusion of a * b + c anywhere in this file: the pair bodies are
 // inlined into several paths (TRA rows, SMS batches, pool rounds, fused and
@@ -1217,7 +1218,7 @@ __global__ void k_init_state(float4 *ev, f3 *acc, int n) {
 
 // FP64 checksum: block b sums slots [b*chunk, (b+1)*chunk) in a fixed tree
 // order; the final block adds the partials in block order.
-constexpr int kSumBlocks = 256;
+constexpr int kSumBlocks = kChecksumBlocks;
 
 __global__ __launch_bounds__(kNT) void k_checksum_partial(const float4 *pos, int n,
                                                           double *partial) {
@@ -1304,6 +1305,35 @@ void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s) {
         hipLaunchKernelGGL(k_integrate<true>, dim3(blocks), dim3(kNT), 0, s, p);
     else
         hipLaunchKernelGGL(k_integrate<false>, dim3(blocks), dim3(kNT), 0, s, p);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation checksum entry point: the per-block partial
+// sums of up to kNoAllocRanges ranges, in a file-scope device array (each GPU
+// holds its own copy for the life of the process).  Host code never passes
+// the array's name to a kernel: it passes the address hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+namespace {
+__device__ __align__(256) double scratch[kNoAllocRanges * kChecksumBlocks * 5];
+
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t checksum_scratch(double **ptr) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 void launch_checksum(const float4 *pos, int n, double *partial, int nblocks,

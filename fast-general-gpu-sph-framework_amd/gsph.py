@@ -154,6 +154,7 @@ _SIGS = {
     "gsph_read_grid": (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "gsph_checksum": (_I, [_P, _P]),
     "gsph_checksum_range": (_I, [_P, _I, _I, _P]),
+    "gsph_checksum_ranges_noalloc": (_I, [_P, _I, _P, _P, _P]),
     "gsph_get_pair_stats": (_I, [_P, C.POINTER(PairStats)]),
     "gsph_kernel_timing": (_I, [_P, _I]),
     "gsph_kernel_times": (_I, [_P, C.c_char_p, _I, _P, _P, _I, C.POINTER(_I)]),
@@ -512,6 +513,15 @@ class System:
         else:
             self._c(self._lib.gsph_checksum_range(self.h, first, count, _ptr(out)))
         return out
+
+    def checksum_ranges_noalloc(self, firsts, counts, out5_device_ptr: int):
+        """gsph_checksum_ranges_noalloc: enqueues the checksums of the ranges
+        into a caller-owned device buffer (5 doubles per range); nothing is
+        allocated and nothing waits (read the buffer after sync())."""
+        f = np.ascontiguousarray(firsts, np.int32)
+        c = np.ascontiguousarray(counts, np.int32)
+        self._c(self._lib.gsph_checksum_ranges_noalloc(self.h, len(f), _ptr(f), _ptr(c),
+                                                       C.c_void_p(out5_device_ptr)))
 
     def pair_stats(self):
         s = PairStats()

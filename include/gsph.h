@@ -328,6 +328,22 @@ int gsph_checksum(gsph_ctx *ctx, double out[5]);
 /* The same checksum over state slots [first, first + count), with i counted
  * from first (the checksum of one scene of a batch). */
 int gsph_checksum_range(gsph_ctx *ctx, int first, int count, double out[5]);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation): the checksum of gsph_checksum_range for nranges ranges
+ * [first[k], first[k] + count[k]) of the current state (say every scene of a
+ * batch) in one call, bit-identical to gsph_checksum_range per range.  The
+ * results go to out5_device, a device buffer of 5 * nranges doubles the caller
+ * provides; first and count are host arrays.  Nothing is allocated and the
+ * host does not wait: the kernels are enqueued on the context's stream (read
+ * out5_device after gsph_sync).  The per-block partial sums live in a static
+ * device array of the library (64 ranges x 256 blocks x 5 doubles, 640 KiB per
+ * GPU, held until the process ends), so nranges <= 64 (GSPH_ERR_CAPACITY
+ * beyond).  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (different contexts or streams) race; each call overwrites all the
+ * partials it reads, so nothing carries over from an earlier call. */
+int gsph_checksum_ranges_noalloc(gsph_ctx *ctx, int nranges, const int *first, const int *count,
+                                 double *out5_device);
 /* Arrangement::getSmsTaskPairStats (sph_arrangement.cu:1595-1636). */
 int gsph_get_pair_stats(gsph_ctx *ctx, gsph_pair_stats *out);
 /* Accepted-pair mask pool: rows (64 words each) taken by the last density
