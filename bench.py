@@ -174,6 +174,30 @@ def reference_scene_leg(gsph, steps=20, warmup=3, no_autotune=False):
     info = s.grid_info()
     s.close()
     n = pos.shape[0]
+    # the reference's SMS build variants (CMakeLists.txt:59-100) under its own
+    # split rule (every dense cell on the SMS path), same launch shape
+    variants = {}
+    v = gsph.System(params, pos)
+    base = v.options()
+    for tag, opts in (("register", {}), ("register_no_local_merge", {"sms_local_merge": 0}),
+                      ("legacy", {"sms_legacy_staging": 1}),
+                      ("register_same_cell_pair_force", {"same_cell_pair_force": 1}),
+                      ("legacy_same_cell_pair_force", {"sms_legacy_staging": 1,
+                                                       "same_cell_pair_force": 1})):
+        o = gsph.Options.from_buffer_copy(base)
+        for k, val in opts.items():
+            setattr(o, k, val)
+        v.set_options(o)
+        v.step(3)
+        v.sync()
+        t0 = time.perf_counter()
+        v.step(10)
+        v.sync()
+        vms = (time.perf_counter() - t0) * 1e3 / 10
+        vst = v.step(3, timings=True)
+        variants[tag] = {"ms_per_step": vms,
+                         "stages_ms": {k: sum(x[k] for x in vst) / len(vst) for k in vst[0]}}
+    v.close()
     return {
         "workload": "the reference's default scene (assets/scene_default.json): 3,944,312 "
                     "particles, h=0.03, ~296 per cell",
@@ -189,6 +213,7 @@ def reference_scene_leg(gsph, steps=20, warmup=3, no_autotune=False):
         "reference_published_hw": "RTX 5090 (README.md:212)",
         "speedup_vs_reference_published": REFERENCE_SCENE_MS / ms,
         "autotuned_options": opt_dict(tuned),
+        "sms_variants_reference_split": variants,
     }
 
 

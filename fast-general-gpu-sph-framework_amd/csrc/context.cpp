@@ -73,6 +73,7 @@ struct gsph_ctx {
     uint32_t *sms_pool = nullptr; // SMS accepted-pair masks, rows of 64 words
     int sms_pool_rows = 0;
     int *sms_off = nullptr;       // first pool row of each SMS item
+    float *scf = nullptr;         // [10][cap_n] same-cell pair sums (option same_cell_pair_force)
     gsph_task *tasks = nullptr;
     int *dc = nullptr;
     SceneGeom *scenes_d = nullptr;
@@ -283,6 +284,9 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.grid_blocks = ctx->grid_blocks;
     p.row_prune = ctx->opt.row_prune;
     p.sms_local_merge = ctx->opt.sms_local_merge;
+    p.sms_legacy = ctx->opt.sms_legacy_staging;
+    p.scf = ctx->opt.same_cell_pair_force ? ctx->scf : nullptr;
+    p.scf_stride = ctx->cap_n;
     p.nbmask = ctx->nbmask;
     p.mask_stride = ctx->cap_n;
     p.use_masks = 0;
@@ -344,6 +348,7 @@ int do_force(gsph_ctx *ctx, bool fuse = false) {
     PhysicsArgs p = physics_args(ctx);
     p.use_masks = ctx->masks_valid && ctx->opt.pair_masks != 0;
     p.fuse_integrate = fuse ? 1 : 0;
+    if (p.scf) { Timed t(ctx, "same_cell"); launch_same_cell(p, ctx->stream); }
     { Timed t(ctx, "force"); launch_force(p, ctx->batch, ctx->stream); }
     return check_launch(ctx, "force");
 }
@@ -497,6 +502,7 @@ size_t buffer_bytes(const gsph_options &o, size_t N, size_t C, size_t S) {
     t += b(rows * 64, 4) + b(task_cap / 2 + 2, 4);                    // sms_pool sms_off
     t += b(task_cap, sizeof(gsph_task)) + b(kCntSlots, 4);            // tasks dc
     t += b(S, sizeof(SceneGeom)) + b(S, sizeof(RelayoutSeg));         // scenes segs
+    if (o.same_cell_pair_force) t += b(10 * N, 4);                    // scf
     t += b(5 * 256, 8) + b(5, 8);                                     // checksum
     return t;
 }
@@ -560,6 +566,8 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->dc, kCntSlots);
     ALLOC(ctx->scenes_d, ctx->cap_s);
     ALLOC(ctx->segs, ctx->cap_s);
+    ctx->scf = nullptr;
+    if (ctx->opt.same_cell_pair_force) ALLOC(ctx->scf, 10 * N);
     ALLOC(ctx->sum_partial, 5 * 256);
     ALLOC(ctx->sum_out, 5);
 #undef ALLOC
@@ -597,6 +605,10 @@ int check_options(gsph_ctx *ctx, gsph_options &o) {
         return fail(ctx, GSPH_ERR_INVALID_ARG, "sms_mask_bytes must be in [0, 65536]");
     if (o.sms_local_merge != 0 && o.sms_local_merge != 1)
         return fail(ctx, GSPH_ERR_INVALID_ARG, "sms_local_merge must be 0 or 1");
+    if (o.sms_legacy_staging != 0 && o.sms_legacy_staging != 1)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "sms_legacy_staging must be 0 or 1");
+    if (o.same_cell_pair_force != 0 && o.same_cell_pair_force != 1)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "same_cell_pair_force must be 0 or 1");
     return GSPH_OK;
 }
 
@@ -1459,6 +1471,11 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
         o.sms_local_merge ^= 1;
         trial(o);
     }
+    {   // the legacy SMS staging on / off (same results; no effect without SMS tasks)
+        gsph_options o = best;
+        o.sms_legacy_staging ^= 1;
+        trial(o);
+    }
     (void)hipFree(pos0);
     (void)hipFree(vel0);
     (void)hipFree(ev0);
@@ -1480,8 +1497,17 @@ int gsph_set_options(gsph_ctx *ctx, const gsph_options *opt) {
     gsph_options o = *opt;
     int rc = check_options(ctx, o);
     if (rc) return rc;
+    if (o.same_cell_pair_force && !ctx->scf && ctx->cap_n > 0) {
+        // the same-cell sums buffer, allocated the first time the option is on
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        rc = dalloc(ctx, &ctx->scf, (size_t)10 * ctx->cap_n);
+        if (rc) return rc;
+    }
     ctx->opt = o;
     set_grid(ctx);
+    // density's SMS masks depend on the staging variant: force tests its
+    // candidates itself until the next density pass
+    ctx->masks_valid = false;
     if (ctx->graph_exec) {
         (void)hipGraphExecDestroy(ctx->graph_exec);
         ctx->graph_exec = nullptr;

@@ -104,6 +104,13 @@ struct PhysicsArgs {
     int grid_blocks;       // persistent grid of density / force
     int row_prune;
     int sms_local_merge;   // isSame pairs of one cell stage their rows once
+    int sms_legacy;        // SMS items stage z-pruned layer segments (legacy staging)
+    // same-cell pair force: per-particle sums of the pairs inside its cell,
+    // 10 floats (pressure xyz, viscosity xyz, gradient xyz, laplacian) with
+    // stride scf_stride, written by k_same_cell for cells of SMS tasks with at
+    // most kSameCellCap particles; null when the option is off
+    float *scf;
+    int scf_stride;
     // accepted-pair masks and row plans of TRA particles: kMaskWords words per
     // entry of compact, word-major with stride mask_stride; written by
     // density, read by force
@@ -156,6 +163,8 @@ void launch_take_flags(int *dc, int *out_host, hipStream_t s);
 void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s);
+constexpr int kSameCellCap = 512;      // particles of a cell the same-cell pass takes
+void launch_same_cell(const PhysicsArgs &p, hipStream_t s);
 constexpr int kChecksumBlocks = 256;   // blocks of the FP64 checksum's first pass
 constexpr int kNoAllocRanges = 64;     // ranges per call of the no-allocation checksum
 constexpr int kMaxDevices = 64;

@@ -4,6 +4,10 @@
 //   --config K                          synthetic workload K (0, 1, 2) instead of a JSON scene
 //   --autotune                          pick launch options and the SMS/TRA split by timing
 //                                       the scene first (no counterpart in the reference)
+// The reference's SMS build options (CMakeLists.txt:59-100) as run-time flags:
+//   --sms-legacy                        GSPH_USE_REGISTER_SMS=OFF: legacy shared staging
+//   --no-sms-local-merge                GSPH_ENABLE_SMS_LOCAL_MERGE=OFF
+//   --same-cell-pair-force              GSPH_ENABLE_SAME_CELL_PAIR_FORCE=ON
 #include <unistd.h>
 
 #include <cstdio>
@@ -39,10 +43,17 @@ int main(int argc, char **argv) {
     int frames = 200;  // headless default (main.cpp:44)
     sph::SceneSource src;
     bool scene_given = false, autotune = false;
+    int sms_legacy = 0, local_merge = 1, same_cell = 0;
     for (int i = 1; i < argc; ++i) {
         const bool has_value = i + 1 < argc;
         if (!std::strcmp(argv[i], "--autotune")) {
             autotune = true;
+        } else if (!std::strcmp(argv[i], "--sms-legacy")) {
+            sms_legacy = 1;
+        } else if (!std::strcmp(argv[i], "--no-sms-local-merge")) {
+            local_merge = 0;
+        } else if (!std::strcmp(argv[i], "--same-cell-pair-force")) {
+            same_cell = 1;
         } else if (has_value &&
                    (!std::strcmp(argv[i], "--headless") || !std::strcmp(argv[i], "--benchmark"))) {
             frames = std::atoi(argv[++i]);
@@ -68,6 +79,17 @@ int main(int argc, char **argv) {
     const sph::float3h side{80.f, 80.f, 80.f};
     sph::HybridSystem system(side, origin, true, src);
     if (!system.ok()) return EXIT_FAILURE;
+    {
+        gsph_options o;
+        if (gsph_get_options(system.context(), &o) != GSPH_OK) return EXIT_FAILURE;
+        o.sms_legacy_staging = sms_legacy;
+        o.sms_local_merge = local_merge;
+        o.same_cell_pair_force = same_cell;
+        if (gsph_set_options(system.context(), &o) != GSPH_OK) {
+            std::fprintf(stderr, "%s\n", gsph_last_error(system.context()));
+            return EXIT_FAILURE;
+        }
+    }
     if (autotune) {
         gsph_options chosen;
         float ms = 0.f;

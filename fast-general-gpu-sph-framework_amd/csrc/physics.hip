@@ -254,7 +254,8 @@ __device__ __forceinline__ bool in_range(float d2, PairRange pr) {
 struct SmsItem {
     bool has_task, active, shared;
     int self, si, gidx, cx, cy, cz, gl, gr;
-    int cell_count;   // particles of the task's cell
+    int cell_begin, cell_count;   // the task's cell
+    int zmin, zmax;               // micro-z bounds of the staged particles (legacy staging)
 };
 
 template <bool kBatch>
@@ -269,19 +270,96 @@ __device__ __forceinline__ SmsItem sms_item(const PhysicsArgs &P, int t0, int T,
     m.active = m.has_task && m.self < cell_end;
     m.si = m.has_task ? min(m.self, cell_end - 1) : 0;
     m.gidx = m.has_task ? scene_of_particle<kBatch>(P.scenes, P.nscenes, tk.cell_begin) : 0;
+    m.cell_begin = m.has_task ? tk.cell_begin : 0;
     m.cell_count = m.has_task ? tk.cell_particle_count : 0;
     m.cx = m.has_task ? tk.cell_pos[0] : 0;
     m.cy = m.has_task ? tk.cell_pos[1] : 0;
     m.cz = m.has_task ? tk.cell_pos[2] : 0;
     int xi = m.has_task ? tk.xxi : 0, xa = m.has_task ? tk.xxx : 0;
-    m.shared = P.sms_local_merge && t0 + 1 < T && P.tasks[t0].cellid == P.tasks[t0 + 1].cellid;
+    // the legacy staging always shares an isSame pair's rows (its iterator
+    // runs over the pair's paired_bounds, kernel_common.cuh:289, 992-1001)
+    m.shared = (P.sms_local_merge || P.sms_legacy) && t0 + 1 < T &&
+               P.tasks[t0].cellid == P.tasks[t0 + 1].cellid;
+    int pb = m.has_task ? tk.paired_bounds : 0;
     if (m.shared) {
         xi = min((int)P.tasks[t0].xxi, (int)P.tasks[t0 + 1].xxi);
         xa = max((int)P.tasks[t0].xxx, (int)P.tasks[t0 + 1].xxx);
+        pb = P.tasks[t0].paired_bounds;   // judgeTask's z bounds of the pair
     }
+    m.zmin = (pb >> 8) & 3;
+    m.zmax = (pb >> 10) & 3;
     m.gl = 4 * m.cx - 4 + xi;
     m.gr = 4 * m.cx + 4 + xa;
     return m;
+}
+
+// Legacy SMS staging (the reference's GSPH_USE_REGISTER_SMS=0 path: the
+// shared-memory iterators SimDenSharedData128 / SimForSharedData128,
+// kernel_common.cuh:282-519 and 886-1151).  A row (dy, dz) below the task's
+// cell (dz = -1) whose x-pruned range holds more than 22 candidates per
+// micro-x layer is read layer by layer, each layer from micro z = zmin on;
+// one above it (dz = +1), each layer up to micro z = zmax.  The slices left
+// out lie five or more micro layers (more than h) away in z from every
+// particle the item stages.  Within a layer the keys ascend in micro z, so
+// a binary search over the sorted keys gives the cut (no dense micro table).
+// Each segment is staged in batches of its own, as the reference's read32Data
+// reads within one segment per batch.  Writes the segments of row r to tab
+// ([lo, hi) pairs, one per layer, by lanes sl < layers) and returns their
+// number; an unpruned row is one segment.  Uniform over the lanes that share
+// the item's staging (the half-wave, or the wave when shared).
+constexpr int kLegacyPerLayer = 22;
+
+// Layers of row r when the legacy staging cuts it, else 0.
+__device__ __forceinline__ int legacy_cut_layers(const SceneGeom &g, const SmsItem &sm, int r,
+                                                 int lo, int hi) {
+    const int dz = r / 3 - 1;
+    const int layers = min(sm.gr, g.gx * 4 - 1) - max(sm.gl, 0) + 1;
+    const bool zcut = (dz < 0 && sm.zmin > 0) || (dz > 0 && sm.zmax < 3);
+    return zcut && hi - lo > kLegacyPerLayer * layers ? layers : 0;
+}
+
+__device__ __forceinline__ int legacy_segments(const PhysicsArgs &P, const SceneGeom &g,
+                                               const SmsItem &sm, int r, int lo, int hi, int sl,
+                                               int *tab) {
+    const int dz = r / 3 - 1;
+    const int gl = max(sm.gl, 0);
+    const int layers = legacy_cut_layers(g, sm, r, lo, hi);
+    wave_sync_lds();   // the previous row's reads of tab precede these writes
+    if (layers == 0) {
+        if (sl == 0) {
+            tab[0] = lo;
+            tab[1] = hi;
+        }
+        wave_sync_lds();
+        return 1;
+    }
+    if (sl < layers) {
+        const int ny = sm.cy + r % 3 - 1, nz = sm.cz + dz;   // a live row (hi > lo)
+        const int gx = gl + sl;
+        const int c = g.cell_base + (gx >> 2) + g.gx * (ny + g.gy * nz);
+        const int b0 = P.begin[c], b1 = P.begin[c + 1];
+        const uint2 xc = P.xcum[c];
+        int a = b0 + layer_count(b0, b1, xc, gx & 3);
+        int e = b0 + layer_count(b0, b1, xc, (gx & 3) + 1);
+        // first key of the layer with micro z >= zt
+        const int zt = dz < 0 ? sm.zmin : sm.zmax + 1;
+        int u = a, v = e;
+        while (u < v) {
+            const int mid = (u + v) >> 1;
+            if ((int)((P.hash[mid] >> 2) & 3u) < zt)
+                u = mid + 1;
+            else
+                v = mid;
+        }
+        if (dz < 0)
+            a = u;
+        else
+            e = u;
+        tab[2 * sl] = a;
+        tab[2 * sl + 1] = e;
+    }
+    wave_sync_lds();
+    return layers;
 }
 
 // ---------------------------------------------------------------------------
@@ -538,7 +616,9 @@ __device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, i
     }
 }
 
-template <bool kBatch, int kStage>
+// kLegacy: SMS items use the legacy staging (P.sms_legacy), a variant of its
+// own so that the default kernels carry none of its registers
+template <bool kBatch, int kStage, bool kLegacy>
 __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     __shared__ float4 s_lds[kNT / 64][DensLds<kStage>::kWords];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -680,6 +760,9 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                     row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
                               min(sm.gr, g.gx * 4 - 1), lo[r], hi[r]);
                 subs += (hi[r] - lo[r] + 31) >> 5;
+                // a cut row's segments each start a sub-batch: at most one more
+                // per layer (an upper bound; force walks the same sub-batches)
+                if (kLegacy) subs += legacy_cut_layers(g, sm, r, lo[r], hi[r]);
             }
             // pool rows of this item: the longer half's 32-candidate sub-batches
             const int rows = max(__builtin_amdgcn_readlane(subs, 0),
@@ -703,25 +786,31 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             }
             float s = 0.f;
             int w = 0;
+            int *tab = reinterpret_cast<int *>(wl + 64) + (sm.shared ? 0 : (lane & 32));
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 const int r = row_of(q);
                 RowSum rs = row_sum_zero();
-                // the next batch's candidate is loaded while this one computes
-                float4 nx = pi;
-                if (sl < min(width, hi[r] - lo[r])) nx = P.pos[lo[r] + sl];
-                for (int b0 = lo[r]; b0 < hi[r]; b0 += width) {
-                    const int cnt = min(width, hi[r] - b0);
-                    if (sl < cnt) lds[sl] = nx;
-                    wave_sync_lds();
-                    if (sl < min(width, hi[r] - b0 - width)) nx = P.pos[b0 + width + sl];
-                    for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
-                        const int n = min(32, cnt - k0);
-                        uint32_t mk = 0;
-                        density_row(pi, lds + k0, n, b0 + k0 - g.p_begin, four, h2, pr, rs, mk);
-                        if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_mask(mk, n);
+                const int nseg = kLegacy ? legacy_segments(P, g, sm, r, lo[r], hi[r], sl, tab) : 1;
+                for (int sg = 0; sg < nseg; ++sg) {
+                    const int slo = kLegacy ? tab[2 * sg] : lo[r];
+                    const int shi = kLegacy ? tab[2 * sg + 1] : hi[r];
+                    // the next batch's candidate is loaded while this one computes
+                    float4 nx = pi;
+                    if (sl < min(width, shi - slo)) nx = P.pos[slo + sl];
+                    for (int b0 = slo; b0 < shi; b0 += width) {
+                        const int cnt = min(width, shi - b0);
+                        if (sl < cnt) lds[sl] = nx;
+                        wave_sync_lds();
+                        if (sl < min(width, shi - b0 - width)) nx = P.pos[b0 + width + sl];
+                        for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
+                            const int n = min(32, cnt - k0);
+                            uint32_t mk = 0;
+                            density_row(pi, lds + k0, n, b0 + k0 - g.p_begin, four, h2, pr, rs, mk);
+                            if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_mask(mk, n);
+                        }
+                        wave_sync_lds();
                     }
-                    wave_sync_lds();
                 }
                 s += row_sum_value(rs, four);
             }
@@ -932,7 +1021,7 @@ __device__ __forceinline__ uint32_t test_row(const float4 &pi, const float4 *q, 
     return finish_mask(mask, cnt);
 }
 
-template <bool kBatch, int kStage>
+template <bool kBatch, int kStage, bool kLegacy>
 __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     __shared__ float4 s_lds[kNT / 64][ForceLds<kStage>::kWords];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1136,7 +1225,21 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             const int sl = sm.shared ? lane : (lane & 31);
             float4 *lp = sm.shared ? wl : wl + (lane & 32);
             float4 *le = lp + 64;
+            int *tab = reinterpret_cast<int *>(wl + 128) + (sm.shared ? 0 : (lane & 32));
             ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            // same-cell pair force: the pairs inside the particle's own cell
+            // come summed from k_same_cell, and its own cell's candidates
+            // (one sorted range inside the centre row) are skipped here
+            // (force_kernels.cu:858-866, 392-398)
+            const bool same_cell = P.scf && sm.has_task && sm.cell_count <= kSameCellCap;
+            if (same_cell && sm.active) {
+                const float *f = P.scf + sm.self;
+                const size_t st = (size_t)P.scf_stride;
+                A = {f[0], f[st], f[2 * st], f[3 * st], f[4 * st], f[5 * st], f[6 * st], f[7 * st],
+                     f[8 * st], f[9 * st]};
+            }
+            const int own_lo = same_cell ? sm.cell_begin : 0;
+            const int own_hi = same_cell ? sm.cell_begin + sm.cell_count : 0;
             for (int q = 0; q < 9; ++q) {
                 const int r = row_of(q);
                 const int ny = sm.cy + r % 3 - 1, nz = sm.cz + r / 3 - 1;
@@ -1144,47 +1247,185 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                 if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
                     row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
                               min(sm.gr, g.gx * 4 - 1), lo, hi);
-                // the next batch's candidate (and its masks) are loaded while
-                // this batch computes
-                float4 np = pi, ne = ei;
-                uint32_t nm0 = 0, nm1 = 0;
-                auto fetch = [&](int b, int wb) __attribute__((always_inline)) {
-                    const int c = min(width, hi - b);
-                    if (sl < c) {
-                        np = P.pos[b + sl];
-                        ne = P.ev[b + sl];
-                    }
-                    if (base >= 0 && c > 0) {
-                        const size_t row0 = (size_t)(base + wb) * 64 + lane;
-                        nm0 = P.sms_pool[row0];
-                        if (c > 32) nm1 = P.sms_pool[row0 + 64];
-                    }
-                };
-                fetch(lo, w);
-                for (int b0 = lo; b0 < hi; b0 += width) {
-                    const int cnt = min(width, hi - b0);
-                    if (sl < cnt) {
-                        lp[sl] = np;
-                        le[sl] = ne;
-                    }
-                    const uint32_t cm0 = nm0, cm1 = nm1;
-                    wave_sync_lds();
-                    fetch(b0 + width, w + ((cnt + 31) >> 5));
-                    // 32 candidates at a time: accept mask, then the pair body
-                    for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
-                        const int n = min(32, cnt - k0);
-                        uint32_t mask = base >= 0 ? (k0 == 0 ? cm0 : cm1) : test_row(pi, lp + k0, n, pr);
-                        while (mask) {
-                            const int kk = k0 + __builtin_ctz(mask);
-                            mask &= mask - 1u;
-                            force_pair(pi, ei, lp[kk], le[kk], P.c, hh, A);
+                const int nseg = kLegacy ? legacy_segments(P, g, sm, r, lo, hi, sl, tab) : 1;
+                for (int sg = 0; sg < nseg; ++sg) {
+                    const int slo = kLegacy ? tab[2 * sg] : lo;
+                    const int shi = kLegacy ? tab[2 * sg + 1] : hi;
+                    // the next batch's candidate (and its masks) are loaded while
+                    // this batch computes
+                    float4 np = pi, ne = ei;
+                    uint32_t nm0 = 0, nm1 = 0;
+                    auto fetch = [&](int b, int wb) __attribute__((always_inline)) {
+                        const int c = min(width, shi - b);
+                        if (sl < c) {
+                            np = P.pos[b + sl];
+                            ne = P.ev[b + sl];
                         }
+                        if (base >= 0 && c > 0) {
+                            const size_t row0 = (size_t)(base + wb) * 64 + lane;
+                            nm0 = P.sms_pool[row0];
+                            if (c > 32) nm1 = P.sms_pool[row0 + 64];
+                        }
+                    };
+                    fetch(slo, w);
+                    for (int b0 = slo; b0 < shi; b0 += width) {
+                        const int cnt = min(width, shi - b0);
+                        if (sl < cnt) {
+                            lp[sl] = np;
+                            le[sl] = ne;
+                        }
+                        const uint32_t cm0 = nm0, cm1 = nm1;
+                        wave_sync_lds();
+                        fetch(b0 + width, w + ((cnt + 31) >> 5));
+                        // 32 candidates at a time: accept mask, then the pair body
+                        for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
+                            const int n = min(32, cnt - k0);
+                            uint32_t mask = base >= 0 ? (k0 == 0 ? cm0 : cm1) : test_row(pi, lp + k0, n, pr);
+                            if (same_cell && r == 4) {
+                                // candidates b0 + k0 + [ol, oh) are the own cell's
+                                const int ol = min(max(own_lo - (b0 + k0), 0), 32);
+                                const int oh = min(max(own_hi - (b0 + k0), 0), 32);
+                                const uint32_t below = ol >= 32 ? ~0u : (1u << ol) - 1u;
+                                const uint32_t upto = oh >= 32 ? ~0u : (1u << oh) - 1u;
+                                mask &= ~(upto & ~below);
+                            }
+                            while (mask) {
+                                const int kk = k0 + __builtin_ctz(mask);
+                                mask &= mask - 1u;
+                                force_pair(pi, ei, lp[kk], le[kk], P.c, hh, A);
+                            }
+                        }
+                        wave_sync_lds();
                     }
-                    wave_sync_lds();
                 }
             }
             if (sm.active) force_store<kBatch>(P, sm.self, pi, ei, A);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Same-cell pair force (the reference's GSPH_ENABLE_SAME_CELL_PAIR_FORCE
+// option: computeSameCellPairForceKernel, force_kernels.cu:9-131).  Every
+// pair of particles inside one SMS cell of at most kSameCellCap particles is
+// computed once and both sides are accumulated; the SMS force pass starts
+// each particle of such a cell from these sums and skips its own cell's
+// candidates.  One workgroup of kSameCellNT threads per cell (the cell of
+// each task with p_offset 0).  Instead of atomics, the pairs are visited in
+// rounds d = 1 .. n/2: in round d particle i pairs with (i + d) mod n (for
+// even n, in round n/2 only i < n/2), which covers each pair once and, within
+// a round, sends the partner's share of every pair to a different particle.
+// A thread keeps the sums of its own particles (i = tid + kSameCellNT u) in
+// registers and adds the partner shares into LDS, the rounds separated by
+// barriers, so the result does not depend on timing.  The terms of each side
+// are those of force_pair.
+constexpr int kSameCellNT = 256;
+constexpr int kSameCellOwn = kSameCellCap / kSameCellNT;
+
+struct SameCellLds {
+    float4 pos[kSameCellCap];
+    float4 ev[kSameCellCap];
+    float part[10][kSameCellCap];   // partner shares
+};
+
+template <int kOwn>
+__device__ __forceinline__ void same_cell_block(const PhysicsArgs &P, SameCellLds &L, int cb, int n,
+                                                HH hh, PairRange pr) {
+    const int tid = threadIdx.x;
+    for (int k = tid; k < n; k += kSameCellNT) {
+        L.pos[k] = P.pos[cb + k];
+        L.ev[k] = P.ev[cb + k];
+#pragma unroll
+        for (int f = 0; f < 10; ++f) L.part[f][k] = 0.f;
+    }
+    __syncthreads();
+    ForceAcc own[kOwn];
+#pragma unroll
+    for (int u = 0; u < kOwn; ++u) own[u] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const gsph_constants &c = P.c;
+    const int half = n >> 1;
+    for (int d = 1; d <= half; ++d) {
+        const int lim = 2 * d == n ? half : n;   // particles that start a pair this round
+#pragma unroll
+        for (int u = 0; u < kOwn; ++u) {
+            const int i = tid + kSameCellNT * u;
+            if (i < lim) {
+                int j = i + d;
+                if (j >= n) j -= n;
+                const float4 pi = L.pos[i], ei = L.ev[i], pj = L.pos[j], ej = L.ev[j];
+                if (in_range(dist2(pi, pj), pr)) {
+                    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                    float d2;
+                    const float t = h2_minus_r2(rx, ry, rz, hh, d2);
+                    const float inv = __frsqrt_rn(d2);
+                    const float kr0 = fmaf(-d2, inv, c.kernel_h);
+                    const float kr =
+                        fmaf(fmaf(-kr0, fmaf(d2, inv, c.kernel_h), t), 0.5f / c.kernel_h, kr0);
+                    const float psum = ei.w + ej.w;
+                    const float lap = fmaf(-0.75f, t, d2);
+                    // side i (V = w_j)
+                    ForceAcc &A = own[u];
+                    const float Vkr = pj.w * kr;
+                    const float pw = Vkr * psum * kr * inv;
+                    A.px = fmaf(-rx, pw, A.px);
+                    A.py = fmaf(-ry, pw, A.py);
+                    A.pz = fmaf(-rz, pw, A.pz);
+                    A.vx = fmaf(ej.x - ei.x, Vkr, A.vx);
+                    A.vy = fmaf(ej.y - ei.y, Vkr, A.vy);
+                    A.vz = fmaf(ej.z - ei.z, Vkr, A.vz);
+                    const float Vt = pj.w * t;
+                    const float gw = Vt * t;
+                    A.gx = fmaf(rx, gw, A.gx);
+                    A.gy = fmaf(ry, gw, A.gy);
+                    A.gz = fmaf(rz, gw, A.gz);
+                    A.l = fmaf(Vt, lap, A.l);
+                    // side j (V = w_i, the relative vectors negated)
+                    const float Wkr = pi.w * kr;
+                    const float qw = Wkr * psum * kr * inv;
+                    const float Wt = pi.w * t;
+                    const float hw = Wt * t;
+                    L.part[0][j] = fmaf(rx, qw, L.part[0][j]);
+                    L.part[1][j] = fmaf(ry, qw, L.part[1][j]);
+                    L.part[2][j] = fmaf(rz, qw, L.part[2][j]);
+                    L.part[3][j] = fmaf(ei.x - ej.x, Wkr, L.part[3][j]);
+                    L.part[4][j] = fmaf(ei.y - ej.y, Wkr, L.part[4][j]);
+                    L.part[5][j] = fmaf(ei.z - ej.z, Wkr, L.part[5][j]);
+                    L.part[6][j] = fmaf(-rx, hw, L.part[6][j]);
+                    L.part[7][j] = fmaf(-ry, hw, L.part[7][j]);
+                    L.part[8][j] = fmaf(-rz, hw, L.part[8][j]);
+                    L.part[9][j] = fmaf(Wt, lap, L.part[9][j]);
+                }
+            }
+        }
+        __syncthreads();   // this round's partner shares precede the next round's
+    }
+#pragma unroll
+    for (int u = 0; u < kOwn; ++u) {
+        const int i = tid + kSameCellNT * u;
+        if (i < n) {
+            const ForceAcc &A = own[u];
+            const float v[10] = {A.px, A.py, A.pz, A.vx, A.vy, A.vz, A.gx, A.gy, A.gz, A.l};
+#pragma unroll
+            for (int f = 0; f < 10; ++f)
+                P.scf[(size_t)f * P.scf_stride + cb + i] = v[f] + L.part[f][i];
+        }
+    }
+    __syncthreads();   // the reads above precede the next cell's staging
+}
+
+__global__ __launch_bounds__(kSameCellNT) void k_same_cell(PhysicsArgs P) {
+    __shared__ SameCellLds L;
+    const int T = P.dc[kCntT];
+    const HH hh = exact_h2(P.c.kernel_h);
+    const PairRange pr = pair_range(P.c.kernel_2);
+    for (int t = blockIdx.x; t < T; t += gridDim.x) {
+        const gsph_task tk = P.tasks[t];
+        const int n = tk.cell_particle_count;
+        if (tk.p_offset != 0 || n > kSameCellCap) continue;   // block-uniform
+        if (n <= kSameCellNT)
+            same_cell_block<1>(P, L, tk.cell_begin, n, hh, pr);
+        else
+            same_cell_block<kSameCellOwn>(P, L, tk.cell_begin, n, hh, pr);
     }
 }
 
@@ -1256,12 +1497,18 @@ __global__ void k_checksum_final(const double *partial, int nblocks, double *out
 
 template <bool B, int S>
 void launch_density_t(const PhysicsArgs &p, hipStream_t s) {
-    hipLaunchKernelGGL((k_density<B, S>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    if (p.sms_legacy)
+        hipLaunchKernelGGL((k_density<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_density<B, S, false>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
 }
 
 template <bool B, int S>
 void launch_force_t(const PhysicsArgs &p, hipStream_t s) {
-    hipLaunchKernelGGL((k_force<B, S>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    if (p.sms_legacy)
+        hipLaunchKernelGGL((k_force<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_force<B, S, false>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
 }
 
 template <bool B>
@@ -1296,6 +1543,11 @@ void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s) {
         launch_force_b<true>(p, s);
     else
         launch_force_b<false>(p, s);
+}
+
+void launch_same_cell(const PhysicsArgs &p, hipStream_t s) {
+    // 36 KB of LDS per block, four blocks per CU; a persistent grid over the tasks
+    hipLaunchKernelGGL(k_same_cell, dim3(1024), dim3(kSameCellNT), 0, s, p);
 }
 
 void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s) {

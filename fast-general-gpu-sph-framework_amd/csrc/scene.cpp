@@ -223,6 +223,8 @@ void gsph_options_default(gsph_options *out) {
     out->stage_depth = 128;
     out->sms_mask_bytes = 1024;
     out->sms_local_merge = 1;
+    out->sms_legacy_staging = 0;
+    out->same_cell_pair_force = 0;
 }
 
 int gsph_derive_constants(const gsph_params *p, gsph_constants *c) {

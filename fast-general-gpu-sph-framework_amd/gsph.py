@@ -74,6 +74,7 @@ class Options(C.Structure):
         ("split_mode", C.c_int), ("sms_min_self", C.c_int), ("sms_min_total", C.c_int),
         ("blocks_per_cu", C.c_int), ("pair_masks", C.c_int), ("row_prune", C.c_int),
         ("stage_depth", C.c_int), ("sms_mask_bytes", C.c_int), ("sms_local_merge", C.c_int),
+        ("sms_legacy_staging", C.c_int), ("same_cell_pair_force", C.c_int),
     ]
 
 

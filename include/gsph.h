@@ -118,8 +118,12 @@ typedef struct gsph_stage_ms {
  * thresholds below (same physics, different TRA/SMS assignment).  The
  * remaining fields are launch-shape knobs; gsph_autotune picks them by
  * timing the context's own scene.  Only split_mode/thresholds change which
- * path a particle takes; every option gives bit-identical results (the TRA
- * and SMS paths sum each particle's neighbours in the same order). */
+ * path a particle takes; every option but same_cell_pair_force gives
+ * bit-identical results (the TRA and SMS paths sum each particle's
+ * neighbours in the same order).  same_cell_pair_force sums the pairs inside
+ * an SMS cell in another order: its results agree with the others within the
+ * FP32 rounding of the sums, not bit for bit, and gsph_autotune never turns
+ * it on. */
 typedef struct gsph_options {
     int split_mode;
     int sms_min_self;     /* tuned: cell is TRA when n <  sms_min_self ... */
@@ -134,6 +138,18 @@ typedef struct gsph_options {
                              stage their shared rows once, in density and force
                              (GSPH_ENABLE_SMS_LOCAL_MERGE and ..._FORCE,
                              CMakeLists.txt:76-91); 0: each task stages its own */
+    int sms_legacy_staging; /* 1: the reference's legacy shared-memory SMS
+                             staging (GSPH_USE_REGISTER_SMS=0,
+                             kernel_common.cuh:282-519, 886-1151): isSame pairs
+                             always share their rows, and dense rows below or
+                             above the cell are read per micro-x layer, cut to
+                             the task's micro-z bounds; same results */
+    int same_cell_pair_force; /* 1: pairs inside an SMS cell of at most 512
+                             particles are computed once by a pass of their
+                             own and both sides accumulated
+                             (GSPH_ENABLE_SAME_CELL_PAIR_FORCE,
+                             force_kernels.cu:9-131, CMakeLists.txt:92-100);
+                             needs 40 B per particle of capacity */
 } gsph_options;
 
 typedef struct gsph_ctx gsph_ctx;

@@ -446,6 +446,144 @@ def test_sms_local_merge_changes_no_bits(gsph, po, name):
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
+def legacy_cut_rows(g, dims):
+    """Rows that the legacy SMS staging reads per micro-x layer cut to the
+    micro-z bounds (kernel_common.cuh:318-356): rows below (above) the task's
+    cell when its z bounds start above 0 (end below 3) and the x-pruned row
+    holds more than 22 candidates per layer, counted from the sorted keys."""
+    gx, gy, gz = dims
+    key = g["hash"].astype(np.int64)
+    lc = np.zeros((gx * gy * gz, 4), np.int64)
+    np.add.at(lc, (key >> 6, (key >> 4) & 3), 1)
+    tasks, T = g["tasks"], g["num_tasks"]
+    cut = 0
+    for k in range(T):
+        t0 = k & ~1
+        shared = t0 + 1 < T and tasks[t0]["cellid"] == tasks[t0 + 1]["cellid"]
+        tk = tasks[k]
+        if shared:
+            xi = min(tasks[t0]["xxi"], tasks[t0 + 1]["xxi"])
+            xa = max(tasks[t0]["xxx"], tasks[t0 + 1]["xxx"])
+            pb = int(tasks[t0]["paired_bounds"])
+        else:
+            xi, xa, pb = tk["xxi"], tk["xxx"], int(tk["paired_bounds"])
+        zmin, zmax = (pb >> 8) & 3, (pb >> 10) & 3
+        cx, cy, cz = (int(v) for v in tk["cell_pos"])
+        gl, gr = max(4 * cx - 4 + int(xi), 0), min(4 * cx + 4 + int(xa), 4 * gx - 1)
+        xs = np.arange(gl, gr + 1)
+        for dz in (-1, 1):
+            if not ((dz < 0 and zmin > 0) or (dz > 0 and zmax < 3)) or not 0 <= cz + dz < gz:
+                continue
+            for dy in (-1, 0, 1):
+                if 0 <= cy + dy < gy:
+                    cnt = lc[(xs >> 2) + gx * (cy + dy + gy * (cz + dz)), xs & 3].sum()
+                    cut += cnt > 22 * len(xs)
+    return cut
+
+
+@pytest.mark.parametrize("name", ["mixed", "dense64", "odd_tasks", "dense"])
+def test_sms_legacy_staging_changes_no_bits(gsph, po, name):
+    """The legacy SMS staging (GSPH_USE_REGISTER_SMS=0: shared rows for every
+    isSame pair, dense rows above and below the cell read per layer within
+    the task's micro-z bounds) on and off: the candidates it leaves out are
+    farther than h from every staged particle and each lane's accepted pairs
+    keep their order, so every bit is the same."""
+    op, pos = scene(po, name)
+    banner(f"sms legacy staging {name}")
+    out = []
+    for legacy in (1, 0):
+        s = make_system(gsph, op, pos, sms_legacy_staging=legacy, sms_local_merge=1 - legacy)
+        s.step(3)
+        out.append(s.read_state())
+        assert s.grid_info()["tasks"] > 0
+        s.close()
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def _reference_block(gsph):
+    from pathlib import Path
+    path = Path(__file__).resolve().parent.parent / "fast-general-gpu-sph-framework_amd" / \
+        "assets" / "scene_default.json"
+    p, pos = gsph.scene_json(path)
+    keep = np.all((pos[:, :3] >= 0.3) & (pos[:, :3] < 0.48), axis=1)
+    return p, np.ascontiguousarray(pos[keep])
+
+
+def test_sms_variants_reference_scene_block(gsph, po):
+    """The reference's SMS variants on a 64,000-particle block of the
+    reference scene under the reference split (about 296 particles per cell,
+    every cell SMS): the legacy staging, which cuts rows there, gives the
+    default's bits; the same-cell pair force (every cell has at most 512
+    particles, so every SMS particle takes its own cell's pairs from the
+    pair pass) stays within REL_TOL of the FP64 oracle, and its pair pass ran
+    (the results are not the default's bits)."""
+    import pyoracle
+    p, blk = _reference_block(gsph)
+    n = len(blk)
+    op = pyoracle.params_from(p)
+    dims = tuple(po.derive(op).grid_size)
+    banner(f"sms variants reference-scene block n={n}")
+    states = {}
+    for tag, opts in (("default", {}), ("legacy", {"sms_legacy_staging": 1}),
+                      ("same_cell", {"same_cell_pair_force": 1})):
+        o = gsph.options_default()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        s = gsph.System(p, blk, options=o)
+        if tag == "legacy":
+            s.arrange()
+            g = s.read_grid()
+            cut = legacy_cut_rows(g, dims)
+            print("legacy cut rows", cut, "of", 6 * g["num_tasks"], flush=True)
+            assert cut > 100
+            s.density()
+            s.force()
+            s.integrate()
+        else:
+            s.step(1)
+        states[tag] = s.read_state()
+        s.close()
+    for k in range(4):
+        assert np.array_equal(states["legacy"][k], states["default"][k]), k
+    assert not np.array_equal(states["same_cell"][3], states["default"][3])
+    ref = po.fp64_step(op, blk, np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32))
+    errs = step_errors(*states["same_cell"], ref, op)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e} > {REL_TOL}"
+
+
+@pytest.mark.parametrize("name", ["dense", "dense64", "odd_tasks", "mixed"])
+def test_same_cell_pair_force_physics(gsph, po, name):
+    """The same-cell pair force (GSPH_ENABLE_SAME_CELL_PAIR_FORCE): each pair
+    inside an SMS cell computed once with both sides accumulated, the SMS
+    force skipping its own cell.  A different summation order, so the bar is
+    the FP64 oracle at REL_TOL after one step; density is untouched (same
+    pressure bits) and the accelerations are not the default's bits (the
+    pair pass ran)."""
+    op, pos = scene(po, name)
+    n = len(pos)
+    banner(f"same-cell pair force {name} n={n}")
+    s = make_system(gsph, op, pos, same_cell_pair_force=1)
+    s.step(1)
+    got = s.read_state()
+    tasks = s.grid_info()["tasks"]
+    s.close()
+    assert tasks > 0
+    ref = po.fp64_step(op, pos, np.zeros((n, 3), np.float32), np.zeros((n, 4), np.float32))
+    errs = step_errors(*got, ref, op)
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= REL_TOL, f"{k} error {v:.3e} > {REL_TOL}"
+    d = make_system(gsph, op, pos)
+    d.step(1)
+    base = d.read_state()
+    d.close()
+    assert np.array_equal(got[2][:, 3], base[2][:, 3])          # pressure: density untouched
+    assert not np.array_equal(got[3], base[3])                  # the pair pass ran
+
+
 @pytest.mark.parametrize("name", ["mixed", "dense"])
 def test_split_changes_no_bits(gsph, po, name):
     """The SMS/TRA split moves no bit: both paths walk the same nine rows in

@@ -197,3 +197,11 @@ def test_engine_bytes_accounting(gsph):
     assert b(0, scenes=9) > b(0)
     with pytest.raises(gsph.GsphError):
         b(65537)
+    # the same-cell pair force keeps 10 floats per particle of capacity
+    off = b(256)
+    o.same_cell_pair_force = 1
+    assert b(256) - off == 40 * n
+    o.same_cell_pair_force = 0
+    o.sms_legacy_staging = 1      # no buffer of its own
+    assert b(256) == gsph.engine_bytes(n, base["cells"], base["scenes"], gsph.options_default()) \
+        - n * (1024 - 256)
