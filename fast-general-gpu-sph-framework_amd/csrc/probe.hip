@@ -26,16 +26,21 @@ __global__ __launch_bounds__(kNT) void k_copy(const float4 *src, float4 *dst, si
 }
 
 // Eight independent FP32 FMA chains per lane, 64 FMAs per iteration: the
-// peak VALU issue rate (calibration reference for the VALU counters).
+// peak VALU issue rate (calibration reference for the VALU counters).  The
+// multiplier and addend are per-lane values, so every FMA reads vector
+// registers only: with a scalar operand (a kernel argument) the FMAs issue at
+// about half the rate that all-VGPR FP32 instructions reach on this GPU.
 __global__ __launch_bounds__(kNT) void k_fma_probe(float *out, int iters, float b, float c) {
     float a[8];
+    const float bl = b + 1e-9f * (float)threadIdx.x;
+    const float cl = c * (float)(1 + (threadIdx.x & 7));
 #pragma unroll
     for (int u = 0; u < 8; ++u) a[u] = (float)(threadIdx.x + u);
     for (int i = 0; i < iters; ++i) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a[u] = fmaf(a[u], b, c);
+            for (int u = 0; u < 8; ++u) a[u] = fmaf(a[u], bl, cl);
         }
     }
     float t = 0.f;

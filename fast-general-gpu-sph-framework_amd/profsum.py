@@ -13,7 +13,9 @@ lanes active per VALU instruction, when their inputs were collected; `traffic`
 converts FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into HBM bytes per launch,
 each scaled by what the same counter reads for the plain copy kernel `k_copy`
 over a buffer of known size (bench.py --probe-copy B).  Output is Markdown
-with one fenced JSON block that bench.py reads back.
+with one fenced JSON block that bench.py reads back.  --only k_a,k_b keeps
+the kernels whose names start with one of the given prefixes (the probe
+kernel of `counters` is always used for the peak).
 """
 from __future__ import annotations
 
@@ -50,6 +52,10 @@ def per_dispatch(path: str) -> dict[str, dict[str, float]]:
     return out
 
 
+def keep(args, kernel: str) -> bool:
+    return not args.only or any(kernel.startswith(p) for p in args.only.split(","))
+
+
 def cmd_stats(args):
     rows = list(csv.DictReader(open(args.csv)))
     print("| kernel | calls | avg (us) | min (us) | max (us) | share of kernel time |")
@@ -57,6 +63,8 @@ def cmd_stats(args):
     data = {}
     for r in rows:
         k = short_name(r["Name"])
+        if not keep(args, k):
+            continue
         calls = int(r["Calls"])
         avg = float(r["AverageNs"]) / 1e3
         data[k] = {"calls": calls, "avg_us": avg}
@@ -87,6 +95,8 @@ def cmd_counters(args):
     print("|---|---:|" + "---:|" * len(names) + "---:|---:|---:|")
     out = {}
     for k in sorted(merged):
+        if not keep(args, k) and not k.startswith("k_fma_probe"):
+            continue
         d = merged[k]
         lanes = lanes_per_cu_cycle(d)
         frac = lanes / peak if lanes is not None and peak else None
@@ -117,6 +127,8 @@ def cmd_traffic(args):
     print("|---|---:|---:|---:|---:|---:|")
     out = {}
     for k in sorted(set(fetch) & set(write)):
+        if not keep(args, k):
+            continue
         fk, wk = fetch[k]["FETCH_SIZE"], write[k]["WRITE_SIZE"]
         rb, wb = fk * 1024 * f_scale, wk * 1024 * w_scale
         out[k] = {"fetch_kib": fk, "write_kib": wk, "read_bytes": rb, "write_bytes": wb,
@@ -147,6 +159,8 @@ def main(argv=None):
     t.add_argument("--probe-fetch", required=True)
     t.add_argument("--probe-write", required=True)
     t.add_argument("--probe-bytes", type=float, required=True)
+    for p in (s, c, t):
+        p.add_argument("--only", default="", help="comma-separated kernel name prefixes")
     args = ap.parse_args(argv)
     {"stats": cmd_stats, "counters": cmd_counters, "traffic": cmd_traffic}[args.cmd](args)
 
