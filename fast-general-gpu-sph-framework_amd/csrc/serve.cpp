@@ -24,6 +24,7 @@
 // Clock: realtime = 1 compares arrival times (s) with the wall clock;
 // realtime = 0 counts arrival in engine iterations (deterministic tests).
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -43,6 +44,7 @@ namespace {
 enum ReqState { kQueued = 0, kActive = 1, kDone = 2, kFailed = 3, kRejected = 4 };
 constexpr int kInFlight = 3;                 // engine steps enqueued ahead of the GPU
 constexpr int kGraphAfter = 4;               // steps on one layout before it is captured
+constexpr int kGraphMinRun = 16;             // ... and steps it is known to have left
 constexpr long long kMaxCells = 1LL << 26;   // micro ids must fit 32 bits
 constexpr long long kMaxParticles = 0x7fffff00LL;
 
@@ -88,6 +90,8 @@ struct gsph_server {
     std::deque<int> pending;      // ring slots of enqueued, unchecked steps
     int ring_next = 0;
     long long since_layout = 0;   // steps enqueued on the current layout
+    long long layout_run = 0;     // steps until the first resident scene finishes
+    bool graphed = false;         // the current layout's step graph exists
     std::string err;
 };
 
@@ -240,6 +244,9 @@ int relayout(gsph_server *s) {
     s->layout = next;
     s->active = next;
     s->since_layout = 0;
+    s->graphed = false;
+    s->layout_run = LLONG_MAX;
+    for (int id : next) s->layout_run = std::min<long long>(s->layout_run, s->reqs[id].left);
     note_peak(s, engine_now(s));
     ++s->recompositions;
     return GSPH_OK;
@@ -530,9 +537,14 @@ int gsph_server_run(gsph_server *s, int realtime) {
             if (rc) return rc;
         }
         if (changed) continue;   // a scene failed: lay out again before stepping
-        // a layout that has lasted a few steps is replayed from a graph
-        // (one launch instead of one per kernel); a re-layout drops it
-        rc = engine_step(s->ctx, s->since_layout >= kGraphAfter);
+        // a layout that has lasted a few steps, and lasts at least kGraphMinRun
+        // more unless an arrival changes it, is replayed from a graph (one
+        // launch instead of one per kernel); a re-layout drops it.  Capturing
+        // and instantiating a graph costs about as much as a few eager steps,
+        // so a layout about to change (a resident scene finishing) stays eager.
+        s->graphed = s->graphed || (s->since_layout >= kGraphAfter &&
+                                    s->layout_run - s->since_layout >= kGraphMinRun);
+        rc = engine_step(s->ctx, s->graphed);
         ++s->since_layout;
         if (rc) return sfail(s, rc, std::string("engine step: ") + gsph_last_error(s->ctx));
         StepRec &sr = s->ring[s->ring_next];
