@@ -254,7 +254,7 @@ def serving_leg(gsph, requests=96, seed=20261207, budget=200e9, load=0.7):
     (1) served capacity at saturation (every request queued at t = 0); (2)
     latency under Poisson arrivals whose offered load is `load` x that
     capacity; (3) the plain loop: each request alone in its own context, one
-    after the other."""
+    after the other, its final state read back as the served results are."""
     import numpy as np
 
     worlds, blocks, steps, rng = gsph.serving_workload(requests, seed)
@@ -278,17 +278,35 @@ def serving_leg(gsph, requests=96, seed=20261207, budget=200e9, load=0.7):
     lam = load * capacity / (work / requests)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, requests))
     lat = serve(arrivals)
-    # plain loop: one plain context per request, back to back
+    # plain loop: one plain context per request, back to back, each result
+    # (pos, vel, ev, the state the server returns) read back into pinned host
+    # buffers allocated beforehand, as the server's are at submission
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+    hip.hipHostFree.argtypes = [ctypes.c_void_p]
+    outs = []
+    for b in blocks:
+        ptrs = []
+        for width in (16, 12, 16):
+            q = ctypes.c_void_p()
+            if hip.hipHostMalloc(ctypes.byref(q), len(b) * width, 0) != 0:
+                raise RuntimeError("hipHostMalloc failed")
+            ptrs.append(q)
+        outs.append(ptrs)
     t0 = time.perf_counter()
-    for w, blk, st in zip(worlds, blocks, steps):
+    for w, blk, st, (op, ov, oe) in zip(worlds, blocks, steps, outs):
         p, _ = gsph.scene_config(0)
         for a in range(3):
             p.world_size[a] = w[a]
         s = gsph.System(p, blk)
         s.step(st)
-        s.sync()
+        gsph._check(s._lib.gsph_read_state(s.h, op, ov, oe, None), s.h)
         s.close()
     loop_ps = work / (time.perf_counter() - t0)
+    for ptrs in outs:
+        for q in ptrs:
+            hip.hipHostFree(q)
     return {
         "workload": f"{requests} requests, N log-uniform 8192..1048576 (total {int(ns.sum())}), "
                     f"steps 50..500, {budget / 1e9:.0f} GB HBM budget",

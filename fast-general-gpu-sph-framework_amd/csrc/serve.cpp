@@ -324,14 +324,9 @@ void harvest(gsph_server *s, bool wait_one, double now) {
             r.state = kFailed;
             r.error = "state left the box or is not finite";
         }
-        if (r.in_pos) {
-            (void)hipHostFree(r.in_pos);
-            r.in_pos = nullptr;
-        }
-        if (r.in_vel) {
-            (void)hipHostFree(r.in_vel);
-            r.in_vel = nullptr;
-        }
+        // the request's pinned input staging stays until the server is
+        // destroyed: freeing pinned memory waits for the whole device, which
+        // would drain the steps in flight
     }
     s->retiring.swap(keep);
 }
