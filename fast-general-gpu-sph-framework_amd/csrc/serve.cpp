@@ -16,8 +16,10 @@
 // are copied to pinned memory after every step and read once that step has
 // completed: a scene whose state left its box or stopped being finite fails
 // alone (found by reading the resident states); a capacity or sort error ends
-// the run.  A finished scene's output is checked on the host before it is
-// reported done.
+// the run.  A finished scene's output is checked on the host (finite, inside
+// its box) by a worker thread, so that the check of a large scene does not
+// hold back the next steps; a scene that fails it is reported failed when
+// the run returns.
 // HBM accounting is the exact byte count of the engine's buffers at its
 // allocated capacity (engine_bytes), including both engines while a grown one
 // replaces the old.
@@ -27,8 +29,10 @@
 #include <climits>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -60,7 +64,51 @@ struct Request {
     float *in_pos = nullptr, *in_vel = nullptr;                        // pinned inputs
     float *out_pos = nullptr, *out_vel = nullptr, *out_ev = nullptr;   // pinned outputs
     hipEvent_t out_done = nullptr;
+    int valid = -1;          // host check of the output: -1 pending, 0 failed, 1 passed
     std::string error;
+};
+
+// Checks finished outputs on a thread of its own: harvest queues request ids,
+// finish() waits for the queue to drain and joins.  The worker writes only
+// Request::valid, which the loop reads after finish().
+class OutputChecker {
+public:
+    template <class F>
+    explicit OutputChecker(F check) : th_([this, check] {
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [this] { return stop_ || !q_.empty(); });
+            if (q_.empty()) return;
+            const int id = q_.front();
+            q_.pop_front();
+            lk.unlock();
+            check(id);
+            lk.lock();
+        }
+    }) {}
+    ~OutputChecker() { finish(); }
+    void push(int id) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            q_.push_back(id);
+        }
+        cv_.notify_one();
+    }
+    void finish() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_one();
+        if (th_.joinable()) th_.join();
+    }
+
+private:
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<int> q_;
+    bool stop_ = false;
+    std::thread th_;
 };
 
 struct StepRec {
@@ -301,8 +349,9 @@ int check_oldest_step(gsph_server *s, bool wait, double now, bool &changed) {
     return GSPH_OK;
 }
 
-// Finishes requests whose output copy has completed.
-void harvest(gsph_server *s, bool wait_one, double now) {
+// Finishes requests whose output copy has completed; their host check runs
+// on the checker thread.
+void harvest(gsph_server *s, bool wait_one, double now, OutputChecker &checker) {
     std::vector<int> keep;
     bool waited = false;
     for (int id : s->retiring) {
@@ -318,12 +367,8 @@ void harvest(gsph_server *s, bool wait_one, double now) {
         }
         r.retiring = false;
         r.finished = now;
-        if (scene_ok(r.out_pos, r.n, r.world)) {
-            r.state = kDone;
-        } else {
-            r.state = kFailed;
-            r.error = "state left the box or is not finite";
-        }
+        r.state = kDone;   // until the checker reports otherwise
+        checker.push(id);
         // the request's pinned input staging stays until the server is
         // destroyed: freeing pinned memory waits for the whole device, which
         // would drain the steps in flight
@@ -461,6 +506,23 @@ int gsph_server_run(gsph_server *s, int realtime) {
         return s->reqs[a].arrival < s->reqs[b].arrival;
     });
     const auto t0 = std::chrono::steady_clock::now();
+    OutputChecker checker([s](int id) {
+        Request &r = s->reqs[id];
+        r.valid = scene_ok(r.out_pos, r.n, r.world) ? 1 : 0;
+    });
+    // applies the checker's verdicts (after it has joined), also on an early return
+    struct Verdicts {
+        gsph_server *s;
+        OutputChecker &c;
+        ~Verdicts() {
+            c.finish();
+            for (auto &r : s->reqs)
+                if (r.state == kDone && r.valid == 0) {
+                    r.state = kFailed;
+                    r.error = "state left the box or is not finite";
+                }
+        }
+    } verdicts{s, checker};
     auto clock = [&]() -> double {
         if (realtime)
             return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -473,7 +535,7 @@ int gsph_server_run(gsph_server *s, int realtime) {
         // 1. completed steps' error flags, finished outputs
         int rc = check_oldest_step(s, false, now, changed);
         if (rc) return rc;
-        harvest(s, false, now);
+        harvest(s, false, now, checker);
         // 2. admission, arrival order, budget / slots / grid permitting
         while (!s->queue.empty()) {
             Request &r = s->reqs[s->queue.front()];
@@ -493,7 +555,7 @@ int gsph_server_run(gsph_server *s, int realtime) {
                 continue;
             }
             if (!s->retiring.empty()) {
-                harvest(s, true, clock());
+                harvest(s, true, clock(), checker);
                 continue;
             }
             if (s->queue.empty()) break;
@@ -572,6 +634,7 @@ int gsph_server_run(gsph_server *s, int realtime) {
         s->active.swap(keep);
     }
     if (s->ctx) (void)hipStreamSynchronize(engine_stream(s->ctx));
+    checker.finish();   // the last checks belong to the run
     s->wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return GSPH_OK;
 }
