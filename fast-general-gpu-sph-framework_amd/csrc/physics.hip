@@ -85,56 +85,54 @@ __device__ __forceinline__ void row_range(const SceneGeom &g, const int *begin,
     (void)cx;
 }
 
-struct RowPlan {
-    int ny[9], nz[9], gl[9], gr[9];
-    bool live[9];
+// Row r (dy = r % 3 - 1, dz = r / 3 - 1) of a particle at (x, y, z) in
+// micro cell (mx, my, mz): cell row (ny, nz) and micro-x layers [gl, gr];
+// returns whether the row is live.  Reference rows: layers [mx - 4, mx + 4].
+// Pruned rows: the slab distance d of the row in (y, z) gives the x reach
+// sqrt(h^2 - d^2); layers outside [x - reach, x + reach] are dropped (with a
+// small relative margin, so that a pair the FP32 test accepts is never
+// pruned).
+struct RowSlack {
+    float sx, sy, sz;
 };
 
-// The 9 rows of a particle at (x, y, z) in micro cell (mx, my, mz).  Reference
-// rows: layers [mx - 4, mx + 4].  Pruned rows: the slab distance d of the row
-// in (y, z) gives the x reach sqrt(h^2 - d^2); layers outside [x - reach,
-// x + reach] are dropped (with a small relative margin, so that a pair the
-// FP32 test accepts is never pruned).
-__device__ __forceinline__ void plan_rows(const SceneGeom &g, const gsph_constants &c,
-                                          float x, float y, float z, int mx,
-                                          int my, int mz, bool prune, RowPlan &rp) {
+// Rounding slack of the slab distances and the reach.  The hash puts a cell
+// boundary where fl(y * rat) crosses a multiple of 4, which differs from
+// fl(cy * h) by a few ulp of y; fl(x -+ reach) rounds by half an ulp of x.
+// 2^-20 of the coordinate (8 ulp) plus the fixed margins of plan_row (1e-3
+// relative on h^2, 1e-3 on the reach, 1e-4 h) keep the pruned set a superset
+// of the pairs the FP32 test accepts for every grid up to 16383 cells per
+// axis.
+__device__ __forceinline__ RowSlack row_slack(float x, float y, float z, float h) {
+    return {(fabsf(x) + h) * 0x1p-20f, (fabsf(y) + h) * 0x1p-20f, (fabsf(z) + h) * 0x1p-20f};
+}
+
+__device__ __forceinline__ bool plan_row(const SceneGeom &g, const gsph_constants &c, float x,
+                                         float y, float z, int mx, int my, int mz, bool prune,
+                                         RowSlack sl, int r, int &ny, int &nz, int &gl, int &gr) {
     const int cy = my >> 2, cz = mz >> 2;
     const float h = c.cell_size;
     const float rat = 4.f * c.inv_cell_size;
-    // Rounding slack of the slab distances and the reach.  The hash puts a
-    // cell boundary where fl(y * rat) crosses a multiple of 4, which differs
-    // from fl(cy * h) by a few ulp of y; fl(x -+ reach) rounds by half an ulp
-    // of x.  2^-20 of the coordinate (8 ulp) plus the fixed margins below
-    // (1e-3 relative on h^2, 1e-3 on the reach, 1e-4 h) keep the pruned set a
-    // superset of the pairs the FP32 test accepts for every grid up to 16383
-    // cells per axis.
-    const float sy = (fabsf(y) + h) * 0x1p-20f, sz = (fabsf(z) + h) * 0x1p-20f;
-    const float sx = (fabsf(x) + h) * 0x1p-20f;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        const int dy = r % 3 - 1, dz = r / 3 - 1;
-        const int ny = cy + dy, nz = cz + dz;
-        rp.ny[r] = ny;
-        rp.nz[r] = nz;
-        rp.live[r] = ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz;
-        int gl = mx - 4, gr = mx + 4;
-        if (prune && rp.live[r]) {
-            const float ey = dy < 0 ? y - (float)cy * h - sy : dy > 0 ? (float)(cy + 1) * h - y - sy : 0.f;
-            const float ez = dz < 0 ? z - (float)cz * h - sz : dz > 0 ? (float)(cz + 1) * h - z - sz : 0.f;
-            const float d2 = fmaxf(ey, 0.f) * fmaxf(ey, 0.f) + fmaxf(ez, 0.f) * fmaxf(ez, 0.f);
-            const float lim = c.kernel_2 * 1.001f;
-            if (d2 > lim) {
-                rp.live[r] = false;
-            } else {
-                const float reach = sqrtf(fmaxf(lim - d2, 0.f)) * 1.001f + 1e-4f * h + sx;
-                gl = max(gl, (int)floorf((x - reach) * rat));
-                gr = min(gr, (int)floorf((x + reach) * rat));
-            }
-        }
-        rp.gl[r] = max(gl, 0);
-        rp.gr[r] = min(gr, g.gx * 4 - 1);
-        if (rp.gl[r] > rp.gr[r]) rp.live[r] = false;
+    const int dy = r % 3 - 1, dz = r / 3 - 1;
+    ny = cy + dy;
+    nz = cz + dz;
+    bool live = ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz;
+    gl = mx - 4;
+    gr = mx + 4;
+    if (prune) {   // uniform; selects below, so the row loads can follow at once
+        const float ey = dy < 0 ? y - (float)cy * h - sl.sy : dy > 0 ? (float)(cy + 1) * h - y - sl.sy : 0.f;
+        const float ez = dz < 0 ? z - (float)cz * h - sl.sz : dz > 0 ? (float)(cz + 1) * h - z - sl.sz : 0.f;
+        const float d2 = fmaxf(ey, 0.f) * fmaxf(ey, 0.f) + fmaxf(ez, 0.f) * fmaxf(ez, 0.f);
+        const float lim = c.kernel_2 * 1.001f;
+        // the hardware square root (about 1 ulp) is well inside the margins
+        const float reach = __builtin_amdgcn_sqrtf(fmaxf(lim - d2, 0.f)) * 1.001f + 1e-4f * h + sl.sx;
+        live = live && !(d2 > lim);
+        gl = max(gl, (int)floorf((x - reach) * rat));
+        gr = min(gr, (int)floorf((x + reach) * rat));
     }
+    gl = max(gl, 0);
+    gr = min(gr, g.gx * 4 - 1);
+    return live && gl <= gr;
 }
 
 struct ItemMap {
@@ -201,23 +199,37 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The reference's pair test 1e-12 <= r^2 <= h^2 as one unsigned range test
-// on bit patterns: r^2 is a sum of squares (never negative, never -0), for
-// non-negative floats the bit patterns order like the values, and NaN and
-// +inf fall outside the range.  The host rejects h^2 < 1e-12.
-struct PairRange {
-    uint32_t lo, span;
+// The pair test, evaluated on the density kernel's own variable
+// t = h^2 - r^2: one FMA chain from h^2, t = fl(fl(fl(h^2 - dx^2) - dy^2) -
+// dz^2), and the pair is accepted when 0 <= t < h^2, one unsigned compare of
+// bit patterns (non-negative floats order like their bits; a negative t, -0
+// and NaN carry the sign bit and fall outside).  This is the reference's
+// 1e-12 <= r^2 <= h^2 (density_kernels.cu:166-171) up to FP32 rounding:
+//  - a pair within about an ulp of h^2 of the kernel edge may fall on the
+//    other side; its weight (h^2 - r^2)^3 is below 1e-20 of the largest;
+//  - t == h^2 exactly (the particle itself, coincident particles, and pairs
+//    whose squared components each lie below half an ulp of h^2) is rejected
+//    where the reference rejects r^2 < 1e-12.
+// Density sums t^3 straight from the test's t.  Every kernel evaluates the
+// test with these roundings (force consumes density's accepted pairs).
+struct PairTest {
+    float h2;
+    uint32_t bits;   // bit pattern of h2
 };
 
-__device__ __forceinline__ PairRange pair_range(float k2) {
-    return {__float_as_uint(kFloatSmall), __float_as_uint(k2) - __float_as_uint(kFloatSmall)};
+__device__ __forceinline__ PairTest pair_test(float k2) { return {k2, __float_as_uint(k2)}; }
+
+__device__ __forceinline__ float kernel_t(const float4 &a, const float4 &b, float h2) {
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    return fmaf(-dz, dz, fmaf(-dy, dy, fmaf(-dx, dx, h2)));
 }
 
-// r^2 with explicit FMAs, so that every kernel evaluates the pair test with
-// the same roundings (the force kernel consumes the density kernel's test).
-__device__ __forceinline__ float dist2(const float4 &a, const float4 &b) {
-    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+__device__ __forceinline__ bool accept_t(float t, PairTest pt) {
+    return __float_as_uint(t) < pt.bits;
+}
+
+__device__ __forceinline__ bool accept(const float4 &a, const float4 &b, PairTest pt) {
+    return accept_t(kernel_t(a, b, pt.h2), pt);
 }
 
 // Accepted-pair masks and row plan of TRA particles (entry idx of compact,
@@ -239,10 +251,6 @@ constexpr int kPoolBaseMax = 1 << 29;
 __device__ __forceinline__ uint32_t finish_mask(uint32_t shifted, int cnt) {
     // bits entered at the bottom in candidate order; reverse into bit k
     return cnt > 0 ? __builtin_bitreverse32(shifted) >> (32 - cnt) : 0u;
-}
-
-__device__ __forceinline__ bool in_range(float d2, PairRange pr) {
-    return __float_as_uint(d2) - pr.lo <= pr.span;
 }
 
 // One SMS work item: tasks t0 = 2q and t0 + 1, one per half-wave.  When both
@@ -369,11 +377,11 @@ __device__ __forceinline__ int legacy_segments(const PhysicsArgs &P, const Scene
 // One candidate of the density sum: rs += (h^2 - r^2)^3 if the pair is
 // accepted (a rejected one adds an exact zero), and its acceptance bit is
 // shifted into mk.
-__device__ __forceinline__ void density_pair(const float4 &a, const float4 &b, float h2,
-                                             PairRange pr, float &rs, uint32_t &mk) {
-    const float d2 = dist2(a, b);
-    const bool ok = in_range(d2, pr);
-    const float t = ok ? h2 - d2 : 0.f;
+__device__ __forceinline__ void density_pair(const float4 &a, const float4 &b, PairTest pt,
+                                             float &rs, uint32_t &mk) {
+    const float t0 = kernel_t(a, b, pt.h2);
+    const bool ok = accept_t(t0, pt);
+    const float t = ok ? t0 : 0.f;
     rs = fmaf(t * t, t, rs);
     mk += mk + (ok ? 1u : 0u);
 }
@@ -400,15 +408,65 @@ __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi,
         const int c = g.cell_base + (mx >> 2) + g.gx * ((my >> 2) + g.gy * (mz >> 2));
         *cell_count = P.begin[c + 1] - P.begin[c];
     }
-    RowPlan rp;
-    plan_rows(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, P.row_prune != 0, rp);
+    // The loads of a batch of rows are all issued before any is used (no
+    // branch between them): begin of the left and right cells with the next
+    // begin, and the one x-layer start each side needs.  Rows that are not
+    // live read a clamped in-range cell and are zeroed.  Three batches of
+    // three rows bound the registers the loads hold.
+    const RowSlack sl = row_slack(pi.x, pi.y, pi.z, P.c.cell_size);
+    const bool prune = P.row_prune != 0;
+    const int *begin = P.begin;
+    const uint32_t *xw = reinterpret_cast<const uint32_t *>(P.xcum);
+    auto batch = [&](int r0, int r1) __attribute__((always_inline)) {
+        int bl0[3], bl1[3], br0[3], br1[3], sel[3];
+        uint32_t xl[3], xr[3];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        lo[r] = hi[r] = 0;
-        if (active && rp.live[r])
-            row_range(g, P.begin, P.xcum, mx >> 2, rp.ny[r], rp.nz[r], rp.gl[r], rp.gr[r],
-                      lo[r], hi[r]);
-    }
+        for (int r = r0; r < r1; ++r) {
+            const int k = r - r0;
+            int ny, nz, gl, gr;
+            const bool live = plan_row(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, prune, sl, r, ny, nz, gl, gr);
+            ny = min(max(ny, 0), g.gy - 1);
+            nz = min(max(nz, 0), g.gz - 1);
+            const int rowc = g.cell_base + g.gx * (ny + g.gy * nz);
+            const int lc = rowc + (gl >> 2), rc = rowc + (gr >> 2);
+            const int Ll = gl & 3, Lr = (gr & 3) + 1;
+            bl0[k] = begin[lc];
+            bl1[k] = begin[lc + 1];
+            br0[k] = begin[rc];
+            br1[k] = begin[rc + 1];
+            xl[k] = xw[2 * lc + (Ll == 3 ? 1 : 0)];
+            xr[k] = xw[2 * rc + (Lr == 3 ? 1 : 0)];
+            // bit-field of the layer start each side reads (offset, width):
+            // layer L = 1: bits 0-15 of word 0, 2: bits 16-31, 3: word 1 (its
+            // low 31 bits: v_bfe takes the width mod 32); L = 0 (left) reads
+            // nothing, L = 4 (right) takes the next begin
+            const int offl = Ll == 2 ? 16 : 0, wl = Ll == 0 ? 0 : Ll == 3 ? 31 : 16;
+            const int offr = Lr == 2 ? 16 : 0, wr = Lr == 3 ? 31 : 16;
+            sel[k] = offl | (wl << 5) | (offr << 11) | (wr << 16) | (Lr == 4 ? 1 << 22 : 0) |
+                     ((active && live) ? 1 << 23 : 0);
+        }
+#pragma unroll
+        for (int r = r0; r < r1; ++r) {
+            const int k = r - r0;
+            const uint32_t f = (uint32_t)sel[k];
+            const int vl = (int)__builtin_amdgcn_ubfe(xl[k], f & 31u, (f >> 5) & 63u);
+            const int vr = (int)__builtin_amdgcn_ubfe(xr[k], (f >> 11) & 31u, (f >> 16) & 63u);
+            const int l = bl0[k] + (bl1[k] == bl0[k] ? 0 : vl);
+            const int h = (f >> 22) & 1u ? br1[k] : br0[k] + (br1[k] == br0[k] ? 0 : vr);
+            const bool on = (f >> 23) & 1u;
+            lo[r] = on ? l : 0;
+            hi[r] = on ? h : 0;
+        }
+    };
+    // each batch's bounds are formed before the next batch's loads issue, so
+    // that its loaded words are dead by then
+    batch(0, 3);
+    asm volatile("" ::"v"(lo[0]), "v"(hi[0]), "v"(lo[1]), "v"(hi[1]), "v"(lo[2]), "v"(hi[2])
+                 : "memory");
+    batch(3, 6);
+    asm volatile("" ::"v"(lo[3]), "v"(hi[3]), "v"(lo[4]), "v"(hi[4]), "v"(lo[5]), "v"(hi[5])
+                 : "memory");
+    batch(6, 9);
 }
 
 // The order in which every path visits a particle's 9 rows (r = 3 (dz + 1)
@@ -585,29 +643,29 @@ __device__ __forceinline__ void chains_rotate(float &a0, float &a1, float &a2, f
 // Candidates q[0, cnt) of a row, the first at place off of its scene; accept
 // mask mk (bits in candidate order enter at the bottom).
 __device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, int cnt, int off,
-                                            bool four, float h2, PairRange pr, RowSum &rs,
+                                            bool four, PairTest pt, RowSum &rs,
                                             uint32_t &mk) {
     if (!four) {
         int k = 0;
         for (; k + 4 <= cnt; k += 4) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], h2, pr, rs.s1, mk);
+            for (int u = 0; u < 4; ++u) density_pair(pi, q[k + u], pt, rs.s1, mk);
         }
-        for (; k < cnt; ++k) density_pair(pi, q[k], h2, pr, rs.s1, mk);
+        for (; k < cnt; ++k) density_pair(pi, q[k], pt, rs.s1, mk);
     } else {
         const int r = off & 3;
         float a0 = rs.a0, a1 = rs.a1, a2 = rs.a2, a3 = rs.a3;
         chains_rotate(a0, a1, a2, a3, r);
         int k = 0;
         for (; k + 4 <= cnt; k += 4) {
-            density_pair(pi, q[k], h2, pr, a0, mk);
-            density_pair(pi, q[k + 1], h2, pr, a1, mk);
-            density_pair(pi, q[k + 2], h2, pr, a2, mk);
-            density_pair(pi, q[k + 3], h2, pr, a3, mk);
+            density_pair(pi, q[k], pt, a0, mk);
+            density_pair(pi, q[k + 1], pt, a1, mk);
+            density_pair(pi, q[k + 2], pt, a2, mk);
+            density_pair(pi, q[k + 3], pt, a3, mk);
         }
-        if (k < cnt) density_pair(pi, q[k], h2, pr, a0, mk);
-        if (k + 1 < cnt) density_pair(pi, q[k + 1], h2, pr, a1, mk);
-        if (k + 2 < cnt) density_pair(pi, q[k + 2], h2, pr, a2, mk);
+        if (k < cnt) density_pair(pi, q[k], pt, a0, mk);
+        if (k + 1 < cnt) density_pair(pi, q[k + 1], pt, a1, mk);
+        if (k + 2 < cnt) density_pair(pi, q[k + 2], pt, a2, mk);
         chains_rotate(a0, a1, a2, a3, (4 - r) & 3);
         rs.a0 = a0;
         rs.a1 = a1;
@@ -631,8 +689,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     const int tra_tiles = (tra + 63) >> 6;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
-    const float h2 = P.c.kernel_2;
-    const PairRange pr = pair_range(h2);
+    const PairTest pt = pair_test(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
         if (it < tra_tiles) {
@@ -712,7 +769,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                             const int nc = ce - cb;
                             uint32_t w = 0;
                             if (nc > 0) {
-                                density_row(pi, stg + (cb - base), nc, cb - pb, four, h2, pr, rs, w);
+                                density_row(pi, stg + (cb - base), nc, cb - pb, four, pt, rs, w);
                                 w = finish_mask(w, nc) << (cb - base - 32 * c);
                             }
                             if (round < pool_rounds)   // always, by the bound
@@ -720,7 +777,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                         }
                         ++round;
                     } else if (cnt > 0) {
-                        density_row(pi, stg + (jb - base), cnt, jb - pb, four, h2, pr, rs, mk);
+                        density_row(pi, stg + (jb - base), cnt, jb - pb, four, pt, rs, mk);
                     }
                     wave_sync_lds();
                 }
@@ -806,7 +863,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                         for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                             const int n = min(32, cnt - k0);
                             uint32_t mk = 0;
-                            density_row(pi, lds + k0, n, b0 + k0 - g.p_begin, four, h2, pr, rs, mk);
+                            density_row(pi, lds + k0, n, b0 + k0 - g.p_begin, four, pt, rs, mk);
                             if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_mask(mk, n);
                         }
                         wave_sync_lds();
@@ -1010,14 +1067,14 @@ struct ForceLds {
 
 // Accept mask of cnt <= 32 staged candidates q[0, cnt): bit k for candidate k.
 __device__ __forceinline__ uint32_t test_row(const float4 &pi, const float4 *q, int cnt,
-                                             PairRange pr) {
+                                             PairTest pt) {
     uint32_t mask = 0;   // bits enter at the bottom (reversed order, undone below)
     int k = 0;
     for (; k + 4 <= cnt; k += 4) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) mask += mask + (in_range(dist2(pi, q[k + u]), pr) ? 1u : 0u);
+        for (int u = 0; u < 4; ++u) mask += mask + (accept(pi, q[k + u], pt) ? 1u : 0u);
     }
-    for (; k < cnt; ++k) mask += mask + (in_range(dist2(pi, q[k]), pr) ? 1u : 0u);
+    for (; k < cnt; ++k) mask += mask + (accept(pi, q[k], pt) ? 1u : 0u);
     return finish_mask(mask, cnt);
 }
 
@@ -1041,7 +1098,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const HH hh = exact_h2(P.c.kernel_h);
-    const PairRange pr = pair_range(P.c.kernel_2);
+    const PairTest pt = pair_test(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
         if (it < tra_tiles) {
@@ -1114,8 +1171,8 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                     const int oa = lo[ra] - L[ra];
                     const int ob = g < 4 ? S[ra] + lo[rb] - L[rb] : 0;
                     if (self_test) {
-                        ma = test_row(pi, bp + oa, hi[ra] - lo[ra], pr);
-                        if (g < 4) mb = test_row(pi, bp + ob, hi[rb] - lo[rb], pr);
+                        ma = test_row(pi, bp + oa, hi[ra] - lo[ra], pt);
+                        if (g < 4) mb = test_row(pi, bp + ob, hi[rb] - lo[rb], pt);
                     }
                     while (ma | mb) {
                         const bool ua = ma != 0;
@@ -1200,7 +1257,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                         // 32 candidates at a time: accept mask, then the pair
                         // body over the accepted ones (in order)
                         for (int b0 = 0; b0 < je - jb; b0 += 32) {
-                            uint32_t mask = test_row(pi, qp + b0, min(32, je - jb - b0), pr);
+                            uint32_t mask = test_row(pi, qp + b0, min(32, je - jb - b0), pt);
                             while (mask) {
                                 const int kk = b0 + __builtin_ctz(mask);
                                 mask &= mask - 1u;
@@ -1280,7 +1337,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
                         // 32 candidates at a time: accept mask, then the pair body
                         for (int k0 = 0; k0 < cnt; k0 += 32, ++w) {
                             const int n = min(32, cnt - k0);
-                            uint32_t mask = base >= 0 ? (k0 == 0 ? cm0 : cm1) : test_row(pi, lp + k0, n, pr);
+                            uint32_t mask = base >= 0 ? (k0 == 0 ? cm0 : cm1) : test_row(pi, lp + k0, n, pt);
                             if (same_cell && r == 4) {
                                 // candidates b0 + k0 + [ol, oh) are the own cell's
                                 const int ol = min(max(own_lo - (b0 + k0), 0), 32);
@@ -1330,7 +1387,7 @@ struct SameCellLds {
 
 template <int kOwn>
 __device__ __forceinline__ void same_cell_block(const PhysicsArgs &P, SameCellLds &L, int cb, int n,
-                                                HH hh, PairRange pr) {
+                                                HH hh, PairTest pt) {
     const int tid = threadIdx.x;
     for (int k = tid; k < n; k += kSameCellNT) {
         L.pos[k] = P.pos[cb + k];
@@ -1353,7 +1410,7 @@ __device__ __forceinline__ void same_cell_block(const PhysicsArgs &P, SameCellLd
                 int j = i + d;
                 if (j >= n) j -= n;
                 const float4 pi = L.pos[i], ei = L.ev[i], pj = L.pos[j], ej = L.ev[j];
-                if (in_range(dist2(pi, pj), pr)) {
+                if (accept(pi, pj, pt)) {
                     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
                     float d2;
                     const float t = h2_minus_r2(rx, ry, rz, hh, d2);
@@ -1417,15 +1474,15 @@ __global__ __launch_bounds__(kSameCellNT) void k_same_cell(PhysicsArgs P) {
     __shared__ SameCellLds L;
     const int T = P.dc[kCntT];
     const HH hh = exact_h2(P.c.kernel_h);
-    const PairRange pr = pair_range(P.c.kernel_2);
+    const PairTest pt = pair_test(P.c.kernel_2);
     for (int t = blockIdx.x; t < T; t += gridDim.x) {
         const gsph_task tk = P.tasks[t];
         const int n = tk.cell_particle_count;
         if (tk.p_offset != 0 || n > kSameCellCap) continue;   // block-uniform
         if (n <= kSameCellNT)
-            same_cell_block<1>(P, L, tk.cell_begin, n, hh, pr);
+            same_cell_block<1>(P, L, tk.cell_begin, n, hh, pt);
         else
-            same_cell_block<kSameCellOwn>(P, L, tk.cell_begin, n, hh, pr);
+            same_cell_block<kSameCellOwn>(P, L, tk.cell_begin, n, hh, pt);
     }
 }
 
