@@ -58,31 +58,76 @@ __device__ __forceinline__ int scene_of_particle(const SceneGeom *sc, int ns,
 
 // #particles of a cell (begin b0, next cell's begin b1, x-layer starts x)
 // whose micro x layer is < L (L in 0..4).  x is only meaningful for an
-// occupied cell; selects instead of branches, so the loads feeding it can
-// all be in flight at once.
+// occupied cell.
 __device__ __forceinline__ int layer_count(int b0, int b1, uint2 x, int L) {
     const int cnt = b1 - b0;
     const int v = L == 1 ? (int)(x.x & 0xffffu) : L == 2 ? (int)(x.x >> 16) : (int)x.y;
     return L <= 0 ? 0 : L >= 4 ? cnt : cnt == 0 ? 0 : v;
 }
 
-// Sorted range of row (ny, nz) for candidates spanning micro-x layers
-// [gl, gr] (global micro-x indices).  Six independent loads: begin and the
-// next begin of the left and right cells, and their x-layer starts.
-__device__ __forceinline__ void row_range(const SceneGeom &g, const int *begin,
-                                          const uint2 *xcum, int cx, int ny,
-                                          int nz, int gl, int gr, int &lo,
-                                          int &hi) {
-    const int rowc = g.cell_base + g.gx * (ny + g.gy * nz);
-    const int lcx = max(gl >> 2, 0);
-    const int rcx = min(gr >> 2, g.gx - 1);
-    const int lc = rowc + lcx, rc = rowc + rcx;
-    const int bl0 = begin[lc], bl1 = begin[lc + 1];
-    const int br0 = begin[rc], br1 = begin[rc + 1];
-    const uint2 xl = xcum[lc], xr = xcum[rc];
-    lo = bl0 + ((gl >> 2) == lcx ? layer_count(bl0, bl1, xl, gl & 3) : 0);
-    hi = br0 + ((gr >> 2) == rcx ? layer_count(br0, br1, xr, (gr & 3) + 1) : br1 - br0);
-    (void)cx;
+// Sorted ranges [lo, hi) of nine rows: plan(r, ny, nz, gl, gr) gives row r's
+// cell row (ny, nz), its micro-x layers [gl, gr] (in range) and whether it is
+// live; rows that are not live, or all rows when !on, get lo = hi = 0.  A row
+// is the left cell's layer gl to the right cell's layer gr
+// (findTraNeighborParticleRange, kernel_common.cuh:69-112).  The loads of a
+// batch of rows are all issued before any is used (no branch between them):
+// begin of the left and right cells with the next begin, and the one x-layer
+// start each side needs; rows that are not live read a clamped in-range cell.
+// Three batches of three rows bound the registers the loads hold, and each
+// batch's bounds are formed before the next batch's loads issue, so that its
+// loaded words are dead by then.
+template <typename Plan>
+__device__ __forceinline__ void row_ranges(const SceneGeom &g, const int *begin, const uint2 *xcum,
+                                           bool on, Plan plan, int lo[9], int hi[9]) {
+    const uint32_t *xw = reinterpret_cast<const uint32_t *>(xcum);
+    auto batch = [&](int r0, int r1) __attribute__((always_inline)) {
+        int bl0[3], bl1[3], br0[3], br1[3], sel[3];
+        uint32_t xl[3], xr[3];
+#pragma unroll
+        for (int r = r0; r < r1; ++r) {
+            const int k = r - r0;
+            int ny, nz, gl, gr;
+            const bool live = plan(r, ny, nz, gl, gr);
+            ny = min(max(ny, 0), g.gy - 1);
+            nz = min(max(nz, 0), g.gz - 1);
+            const int rowc = g.cell_base + g.gx * (ny + g.gy * nz);
+            const int lc = rowc + (gl >> 2), rc = rowc + (gr >> 2);
+            const int Ll = gl & 3, Lr = (gr & 3) + 1;
+            bl0[k] = begin[lc];
+            bl1[k] = begin[lc + 1];
+            br0[k] = begin[rc];
+            br1[k] = begin[rc + 1];
+            xl[k] = xw[2 * lc + (Ll == 3 ? 1 : 0)];
+            xr[k] = xw[2 * rc + (Lr == 3 ? 1 : 0)];
+            // bit-field of the layer start each side reads (offset, width):
+            // layer L = 1: bits 0-15 of word 0, 2: bits 16-31, 3: word 1 (its
+            // low 31 bits: v_bfe takes the width mod 32); L = 0 (left) reads
+            // nothing, L = 4 (right) takes the next begin
+            const int offl = Ll == 2 ? 16 : 0, wl = Ll == 0 ? 0 : Ll == 3 ? 31 : 16;
+            const int offr = Lr == 2 ? 16 : 0, wr = Lr == 3 ? 31 : 16;
+            sel[k] = offl | (wl << 5) | (offr << 11) | (wr << 16) | (Lr == 4 ? 1 << 22 : 0) |
+                     ((on && live) ? 1 << 23 : 0);
+        }
+#pragma unroll
+        for (int r = r0; r < r1; ++r) {
+            const int k = r - r0;
+            const uint32_t f = (uint32_t)sel[k];
+            const int vl = (int)__builtin_amdgcn_ubfe(xl[k], f & 31u, (f >> 5) & 63u);
+            const int vr = (int)__builtin_amdgcn_ubfe(xr[k], (f >> 11) & 31u, (f >> 16) & 63u);
+            const int l = bl0[k] + (bl1[k] == bl0[k] ? 0 : vl);
+            const int h = (f >> 22) & 1u ? br1[k] : br0[k] + (br1[k] == br0[k] ? 0 : vr);
+            const bool live = (f >> 23) & 1u;
+            lo[r] = live ? l : 0;
+            hi[r] = live ? h : 0;
+        }
+    };
+    batch(0, 3);
+    asm volatile("" ::"v"(lo[0]), "v"(hi[0]), "v"(lo[1]), "v"(hi[1]), "v"(lo[2]), "v"(hi[2])
+                 : "memory");
+    batch(3, 6);
+    asm volatile("" ::"v"(lo[3]), "v"(hi[3]), "v"(lo[4]), "v"(hi[4]), "v"(lo[5]), "v"(hi[5])
+                 : "memory");
+    batch(6, 9);
 }
 
 // Row r (dy = r % 3 - 1, dz = r / 3 - 1) of a particle at (x, y, z) in
@@ -317,6 +362,21 @@ __device__ __forceinline__ SmsItem sms_item(const PhysicsArgs &P, int t0, int T,
 // the item's staging (the half-wave, or the wave when shared).
 constexpr int kLegacyPerLayer = 22;
 
+// The nine rows of an SMS task: the task's cell row +-1 in y and z, micro-x
+// layers [gl, gr] of the task's (or the merged pair's) particles.
+__device__ __forceinline__ void sms_rows(const SceneGeom &g, const PhysicsArgs &P, const SmsItem &sm,
+                                         int lo[9], int hi[9]) {
+    row_ranges(g, P.begin, P.xcum, sm.has_task,
+               [&](int r, int &ny, int &nz, int &gl, int &gr) __attribute__((always_inline)) {
+                   ny = sm.cy + r % 3 - 1;
+                   nz = sm.cz + r / 3 - 1;
+                   gl = max(sm.gl, 0);
+                   gr = min(sm.gr, g.gx * 4 - 1);
+                   return ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz;
+               },
+               lo, hi);
+}
+
 // Layers of row r when the legacy staging cuts it, else 0.
 __device__ __forceinline__ int legacy_cut_layers(const SceneGeom &g, const SmsItem &sm, int r,
                                                  int lo, int hi) {
@@ -408,65 +468,13 @@ __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi,
         const int c = g.cell_base + (mx >> 2) + g.gx * ((my >> 2) + g.gy * (mz >> 2));
         *cell_count = P.begin[c + 1] - P.begin[c];
     }
-    // The loads of a batch of rows are all issued before any is used (no
-    // branch between them): begin of the left and right cells with the next
-    // begin, and the one x-layer start each side needs.  Rows that are not
-    // live read a clamped in-range cell and are zeroed.  Three batches of
-    // three rows bound the registers the loads hold.
     const RowSlack sl = row_slack(pi.x, pi.y, pi.z, P.c.cell_size);
     const bool prune = P.row_prune != 0;
-    const int *begin = P.begin;
-    const uint32_t *xw = reinterpret_cast<const uint32_t *>(P.xcum);
-    auto batch = [&](int r0, int r1) __attribute__((always_inline)) {
-        int bl0[3], bl1[3], br0[3], br1[3], sel[3];
-        uint32_t xl[3], xr[3];
-#pragma unroll
-        for (int r = r0; r < r1; ++r) {
-            const int k = r - r0;
-            int ny, nz, gl, gr;
-            const bool live = plan_row(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, prune, sl, r, ny, nz, gl, gr);
-            ny = min(max(ny, 0), g.gy - 1);
-            nz = min(max(nz, 0), g.gz - 1);
-            const int rowc = g.cell_base + g.gx * (ny + g.gy * nz);
-            const int lc = rowc + (gl >> 2), rc = rowc + (gr >> 2);
-            const int Ll = gl & 3, Lr = (gr & 3) + 1;
-            bl0[k] = begin[lc];
-            bl1[k] = begin[lc + 1];
-            br0[k] = begin[rc];
-            br1[k] = begin[rc + 1];
-            xl[k] = xw[2 * lc + (Ll == 3 ? 1 : 0)];
-            xr[k] = xw[2 * rc + (Lr == 3 ? 1 : 0)];
-            // bit-field of the layer start each side reads (offset, width):
-            // layer L = 1: bits 0-15 of word 0, 2: bits 16-31, 3: word 1 (its
-            // low 31 bits: v_bfe takes the width mod 32); L = 0 (left) reads
-            // nothing, L = 4 (right) takes the next begin
-            const int offl = Ll == 2 ? 16 : 0, wl = Ll == 0 ? 0 : Ll == 3 ? 31 : 16;
-            const int offr = Lr == 2 ? 16 : 0, wr = Lr == 3 ? 31 : 16;
-            sel[k] = offl | (wl << 5) | (offr << 11) | (wr << 16) | (Lr == 4 ? 1 << 22 : 0) |
-                     ((active && live) ? 1 << 23 : 0);
-        }
-#pragma unroll
-        for (int r = r0; r < r1; ++r) {
-            const int k = r - r0;
-            const uint32_t f = (uint32_t)sel[k];
-            const int vl = (int)__builtin_amdgcn_ubfe(xl[k], f & 31u, (f >> 5) & 63u);
-            const int vr = (int)__builtin_amdgcn_ubfe(xr[k], (f >> 11) & 31u, (f >> 16) & 63u);
-            const int l = bl0[k] + (bl1[k] == bl0[k] ? 0 : vl);
-            const int h = (f >> 22) & 1u ? br1[k] : br0[k] + (br1[k] == br0[k] ? 0 : vr);
-            const bool on = (f >> 23) & 1u;
-            lo[r] = on ? l : 0;
-            hi[r] = on ? h : 0;
-        }
-    };
-    // each batch's bounds are formed before the next batch's loads issue, so
-    // that its loaded words are dead by then
-    batch(0, 3);
-    asm volatile("" ::"v"(lo[0]), "v"(hi[0]), "v"(lo[1]), "v"(hi[1]), "v"(lo[2]), "v"(hi[2])
-                 : "memory");
-    batch(3, 6);
-    asm volatile("" ::"v"(lo[3]), "v"(hi[3]), "v"(lo[4]), "v"(hi[4]), "v"(lo[5]), "v"(hi[5])
-                 : "memory");
-    batch(6, 9);
+    row_ranges(g, P.begin, P.xcum, active,
+               [&](int r, int &ny, int &nz, int &gl, int &gr) __attribute__((always_inline)) {
+                   return plan_row(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, prune, sl, r, ny, nz, gl, gr);
+               },
+               lo, hi);
 }
 
 // The order in which every path visits a particle's 9 rows (r = 3 (dz + 1)
@@ -809,13 +817,9 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
             const int sl = sm.shared ? lane : (lane & 31);
             float4 *lds = sm.shared ? wl : wl + (lane & 32);
             int lo[9], hi[9], subs = 0;
+            sms_rows(g, P, sm, lo, hi);
 #pragma unroll
             for (int r = 0; r < 9; ++r) {
-                const int ny = sm.cy + r % 3 - 1, nz = sm.cz + r / 3 - 1;
-                lo[r] = hi[r] = 0;
-                if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
-                    row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
-                              min(sm.gr, g.gx * 4 - 1), lo[r], hi[r]);
                 subs += (hi[r] - lo[r] + 31) >> 5;
                 // a cut row's segments each start a sub-batch: at most one more
                 // per layer (an upper bound; force walks the same sub-batches)
@@ -1297,13 +1301,22 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
             }
             const int own_lo = same_cell ? sm.cell_begin : 0;
             const int own_hi = same_cell ? sm.cell_begin + sm.cell_count : 0;
+            // the nine rows' bounds, parked in this lane's LDS slots (after
+            // the batch and the legacy segment table) for the row loop
+            static_assert(ForceLds<kStage>::kWords >= 160 + 2 * 9 * 64 / 4, "SMS row slots fit");
+            int *srow = reinterpret_cast<int *>(wl + 160);
+            {
+                int lo9[9], hi9[9];
+                sms_rows(g, P, sm, lo9, hi9);
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    srow[r * 64 + lane] = lo9[r];
+                    srow[(9 + r) * 64 + lane] = hi9[r];
+                }
+            }
             for (int q = 0; q < 9; ++q) {
                 const int r = row_of(q);
-                const int ny = sm.cy + r % 3 - 1, nz = sm.cz + r / 3 - 1;
-                int lo = 0, hi = 0;
-                if (sm.has_task && ny >= 0 && ny < g.gy && nz >= 0 && nz < g.gz)
-                    row_range(g, P.begin, P.xcum, sm.cx, ny, nz, max(sm.gl, 0),
-                              min(sm.gr, g.gx * 4 - 1), lo, hi);
+                const int lo = srow[r * 64 + lane], hi = srow[(9 + r) * 64 + lane];
                 const int nseg = kLegacy ? legacy_segments(P, g, sm, r, lo, hi, sl, tab) : 1;
                 for (int sg = 0; sg < nseg; ++sg) {
                     const int slo = kLegacy ? tab[2 * sg] : lo;
