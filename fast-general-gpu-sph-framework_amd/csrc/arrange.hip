@@ -763,10 +763,14 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
             kcur = a.chunk_k0[q];          // written by k_cells
     }
     // empty chunk (most of a sparsely filled box): begin is the next
-    // occupied cell's start - no map, no barriers
+    // occupied cell's start - no map, no barriers; and nothing to write when
+    // the last fill left the whole chunk at that value already (the region
+    // before the first and after the last occupied cell, step after step)
     const int chunk_end = min(q * kFillChunk + kFillChunk, a.numc);
+    if (q == a.nchunks - 1 && tid == 0) a.begin[a.numc] = a.n;
     if (kcur >= K || a.cells[kcur] >= chunk_end) {
         const int bg = kcur < K ? a.cstart[kcur] : a.n;
+        if (a.fill_val[q] == bg) return;   // block-uniform
         const int4 b4 = make_int4(bg, bg, bg, bg);
         for (int c0 = q * kFillChunk + 4 * tid; c0 < chunk_end; c0 += 4 * kNT) {
             if (c0 + 3 < a.numc) {
@@ -775,9 +779,11 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
                 for (int u = 0; u < 4 && c0 + u < a.numc; ++u) a.begin[c0 + u] = bg;
             }
         }
-        if (q == a.nchunks - 1 && tid == 0) a.begin[a.numc] = a.n;
+        __syncthreads();   // every thread has read fill_val[q] before it changes
+        if (tid == 0) a.fill_val[q] = bg;
         return;
     }
+    if (tid == 0) a.fill_val[q] = -1;   // read above by this block only
     for (int it = 0; it < kFillChunk / kRound; ++it) {
         const int base = q * kFillChunk + it * kRound;
         if (base >= a.numc) break;
@@ -827,7 +833,6 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
         kcur += (int)s_pre[kRound / 32];
         __syncthreads();
     }
-    if (q == a.nchunks - 1 && tid == 0) a.begin[a.numc] = a.n;
 }
 
 // knCalculateBlockRequirementHybridMode (src/io/gpu_model.cu:137-181): SMS

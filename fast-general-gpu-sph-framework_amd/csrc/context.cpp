@@ -65,7 +65,7 @@ struct gsph_ctx {
     unsigned long long *mflag = nullptr, *status = nullptr;
     uint32_t *mpre = nullptr, *tile_m = nullptr, *tile_occ = nullptr, *dhist = nullptr;
     uint32_t *bcnt = nullptr, *split_sum = nullptr;
-    int *cells = nullptr, *cstart = nullptr, *chunk_k0 = nullptr;
+    int *cells = nullptr, *cstart = nullptr, *chunk_k0 = nullptr, *fill_val = nullptr;
     int *begin = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
     uint2 *xcum = nullptr;
     int *compact = nullptr;
@@ -247,6 +247,7 @@ ArrangeArgs arrange_args(gsph_ctx *ctx) {
     a.cstart = ctx->cstart;
     a.nchunks = ctx->nchunks;
     a.chunk_k0 = ctx->chunk_k0;
+    a.fill_val = ctx->fill_val;
     a.begin = ctx->begin;
     a.xcum = ctx->xcum;
     a.req = ctx->req;
@@ -444,6 +445,9 @@ int configure_layout(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n) {
     ctx->nchunks = std::max(1, (ctx->numc + kFillChunk - 1) / kFillChunk);
     ctx->split_tiles = std::max(1, (n + kSplitTile - 1) / kSplitTile);
     ctx->task_cap = (n + 31) / 32 + std::min(ctx->numc, n) + 2;
+    // a new layout: the fill writes every chunk again (a chunk's extent and
+    // the meaning of its cells may have changed)
+    HIP_TRY(ctx, hipMemsetAsync(ctx->fill_val, 0xff, sizeof(int) * (ctx->nchunks + 1), ctx->stream));
     if (!scenes.empty()) {
         HIP_TRY(ctx, hipMemcpyAsync(ctx->scenes_d, ctx->scenes.data(),
                                     sizeof(SceneGeom) * ctx->scenes.size(),
@@ -496,7 +500,7 @@ size_t buffer_bytes(const gsph_options &o, size_t N, size_t C, size_t S) {
     t += b(4 * 256 * (size_t)msort_tiles((int)N), 8);                 // status
     t += b(N, 12);                                                    // acc
     t += b(cell_tiles, 4) + b(3 * split_tiles, 4);                    // bcnt split_sum
-    t += 2 * b(N + 1, 4) + b(nchunks + 1, 4);                         // cells cstart chunk_k0
+    t += 2 * b(N + 1, 4) + 2 * b(nchunks + 1, 4);                     // cells cstart chunk_k0 fill_val
     t += b(C + 1, 4) + 3 * b(C, 4) + b(C, 8);                         // begin req tra sms xcum
     t += b(N, 4) + b((size_t)kMaskWords * N, 4);                      // compact nbmask
     t += b(rows * 64, 4) + b(task_cap / 2 + 2, 4);                    // sms_pool sms_off
@@ -550,6 +554,7 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->cells, N + 1);
     ALLOC(ctx->cstart, N + 1);
     ALLOC(ctx->chunk_k0, nchunks + 1);
+    ALLOC(ctx->fill_val, nchunks + 1);
     ALLOC(ctx->begin, C + 1);
     ALLOC(ctx->req, C);
     ALLOC(ctx->tra_base, C);
@@ -579,6 +584,8 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
                                 sizeof(unsigned long long) * 4 * 256 * (size_t)msort_tiles((int)N),
                                 ctx->stream));
     HIP_TRY(ctx, hipMemsetAsync(ctx->tasks, 0, sizeof(gsph_task) * task_cap, ctx->stream));
+    // no chunk of begin[] is known uniform yet: the first fill writes every chunk
+    HIP_TRY(ctx, hipMemsetAsync(ctx->fill_val, 0xff, sizeof(int) * (nchunks + 1), ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return GSPH_OK;
 }
@@ -803,6 +810,9 @@ int configure_layout_async(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n,
     ctx->nchunks = std::max(1, (ctx->numc + kFillChunk - 1) / kFillChunk);
     ctx->split_tiles = std::max(1, (n + kSplitTile - 1) / kSplitTile);
     ctx->task_cap = (n + 31) / 32 + std::min(ctx->numc, n) + 2;
+    // a new layout: the fill writes every chunk again (a chunk's extent and
+    // the meaning of its cells may have changed)
+    HIP_TRY(ctx, hipMemsetAsync(ctx->fill_val, 0xff, sizeof(int) * (ctx->nchunks + 1), ctx->stream));
     if (ctx->graph_exec) {
         (void)hipGraphExecDestroy(ctx->graph_exec);
         ctx->graph_exec = nullptr;

@@ -69,6 +69,8 @@ struct ArrangeArgs {
     int *cstart;           // [n+1] first sorted particle of each occupied cell
     int nchunks;
     int *chunk_k0;         // [nchunks] first occupied cell index of each chunk
+    int *fill_val;         // [nchunks] the value every begin[] entry of a chunk holds
+                           // when the last fill wrote it uniform (no occupied cell), else -1
     int *begin;            // [numc+1]; count of cell c = begin[c+1] - begin[c]
     uint2 *xcum;           // [numc] x-layer starts (valid where count > 0)
     int *req;              // [numc] valid where count > 0
