@@ -847,14 +847,21 @@ __device__ __forceinline__ int cell_req(const ArrangeArgs &a, int c, int n) {
     const int nc = slice * g.gz;
     const int x = local % g.gx;
     const int y = (local / g.gx) % g.gy;
-    auto count = [&](int d) { return a.begin[d + 1] - a.begin[d]; };
+    // the six face neighbours' counts: all twelve loads issued at once (a
+    // neighbour outside the grid reads the cell itself and adds nothing)
+    const int nb[6] = {c - 1, c + 1, c - g.gx, c + g.gx, c - slice, c + slice};
+    const bool in[6] = {x > 0, x + 1 < g.gx, y > 0, y + 1 < g.gy, local >= slice,
+                        local + slice < nc};
+    int b0[6], b1[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const int d = in[j] ? nb[j] : c;
+        b0[j] = a.begin[d];
+        b1[j] = a.begin[d + 1];
+    }
     int tot = n;
-    if (x > 0) tot += count(c - 1);
-    if (x + 1 < g.gx) tot += count(c + 1);
-    if (y > 0) tot += count(c - g.gx);
-    if (y + 1 < g.gy) tot += count(c + g.gx);
-    if (local >= slice) tot += count(c - slice);
-    if (local + slice < nc) tot += count(c + slice);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) tot += in[j] ? b1[j] - b0[j] : 0;
     const int min_self = a.split_mode ? a.sms_min_self : 15;
     const int min_tot = a.split_mode ? a.sms_min_total : 96;
     if (tot < min_tot && n < min_self) return 0;
@@ -895,16 +902,30 @@ __global__ __launch_bounds__(kNT) void k_split_count(ArrangeArgs a) {
     __shared__ uint32_t s_tmp[8];
     const int K = a.dc[kCntK];
     if (blockIdx.x * kSplitTile >= K) return;
-    uint32_t st = 0, ss = 0, sr = 0;
-    for (int s = 0; s < kSplitTile / kNT; ++s) {
+    // this thread's cells k = tile + s * kNT + tid: their list entries are
+    // loaded first, all at once, then each cell's layer starts and face
+    // neighbours (cells past K compute on cell 0 and store nothing)
+    constexpr int kPer = kSplitTile / kNT;
+    int cc[kPer], pbs[kPer], ns[kPer];
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
         const int k = blockIdx.x * kSplitTile + s * kNT + threadIdx.x;
-        if (k >= K) break;
-        const int c = a.cells[k];
-        const int pb = a.cstart[k];
-        const int n = a.cstart[k + 1] - pb;
-        a.xcum[c] = x_layer_starts(a.hash, pb, n);
+        const bool v = k < K;
+        cc[s] = v ? a.cells[k] : 0;
+        pbs[s] = v ? a.cstart[k] : 0;
+        ns[s] = v ? a.cstart[k + 1] - pbs[s] : 0;
+    }
+    uint32_t st = 0, ss = 0, sr = 0;
+#pragma unroll
+    for (int s = 0; s < kPer; ++s) {
+        const int k = blockIdx.x * kSplitTile + s * kNT + threadIdx.x;
+        const int c = cc[s], pb = pbs[s], n = ns[s];
+        const uint2 xc = x_layer_starts(a.hash, pb, n);
         const int r = cell_req<kBatch>(a, c, n);
-        a.req[c] = r;
+        if (k < K) {
+            a.xcum[c] = xc;
+            a.req[c] = r;
+        }
         const int nsms = min(n, r * kSmsTaskParticles);
         st += (uint32_t)(n - nsms);
         ss += (uint32_t)nsms;
