@@ -31,6 +31,7 @@ enum CounterSlot {
     kCntMiddle = 4,     // reference d_middle_value_ (-1 without SMS)
     kCntErr = 5,
     kCntSmsPool = 6,    // rows of the SMS mask pool taken this step
+    kCntAnyDefer = 7,   // the arrange epoch in which the lean density pass last left a tile
     kCntTicket = 8,     // 8..11: tile tickets of the mover radix passes 0..3
     kCntMovers = 12,    // particles whose key changed this step
     kCntOrder = 13,     // 1: the slot order is the order of hash[] (set by integrate)

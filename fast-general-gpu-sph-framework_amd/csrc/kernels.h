@@ -127,6 +127,8 @@ struct PhysicsArgs {
     uint32_t *sms_pool;
     int sms_pool_rows;
     int *sms_off;          // [task_cap / 2 + 1]
+    uint8_t *tile_defer;   // [ceil(n / 64)] 1 for the TRA tiles the lean density pass
+                           // leaves to the general one (written every density pass)
     int *dcw;              // writable alias of dc
     float wall_margin;     // the integrator's wall margin h + 0.01: the reference adds
                            // the double literal to the float h and stores the sum as a

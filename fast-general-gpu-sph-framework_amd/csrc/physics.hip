@@ -682,6 +682,171 @@ __device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, i
     }
 }
 
+// One TRA tile of the density pass: 64 consecutive entries of compact[0,
+// tra), one particle per lane.  kLean (the lean pass) takes a tile only when
+// no lane sums in four chains and, with pair masks on, no row exceeds one
+// mask word (no pool rounds); it marks every tile it leaves in tile_defer for
+// the general pass.  Without those paths the lean pass fits 80
+// registers (six waves per SIMD instead of five); both give the same bits.
+// A tile the lean pass leaves: flagged for the general pass, which takes
+// TRA tiles at all only in an arrange epoch where some tile was left (the
+// epoch is stored once, not by every such tile).
+__device__ __forceinline__ void defer_tile(const PhysicsArgs &P, int it, int lane) {
+    if (lane == 0) {
+        P.tile_defer[it] = 1;
+        const int ep = P.dc[kCntEpoch];
+        if (*reinterpret_cast<const volatile int *>(&P.dc[kCntAnyDefer]) != ep) P.dcw[kCntAnyDefer] = ep;
+    }
+}
+
+template <bool kBatch, int kStage, bool kLean>
+__device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int tra, int lane,
+                                             float4 *wl, PairTest pt) {
+    // round walk: row bounds (ints) then the staging round
+    int *rlo = reinterpret_cast<int *>(wl), *rhi = rlo + 9 * 64;
+    float4 *stg = wl + 2 * 9 * 64 / 4;
+        // all lanes stay in the wave (the row walk is cooperative);
+        // lanes past the TRA range carry empty rows
+        const int idx = it * 64 + lane;
+        const bool active = idx < tra;
+        const int i = P.compact[active ? idx : tra - 1];
+        const float4 pi = P.pos[i];
+        if (kLean) {
+            // a lane in a dense cell (four chains) leaves the tile to the
+            // general pass before any row is planned
+            int mx, my, mz;
+            const SceneGeom g = P.scenes[scene_of_particle<kBatch>(P.scenes, P.nscenes, i)];
+            micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
+            const int c = g.cell_base + (mx >> 2) + g.gx * ((my >> 2) + g.gy * (mz >> 2));
+            if (__ballot(active && P.begin[c + 1] - P.begin[c] > kFourChainCell) != 0) {
+                defer_tile(P, it, lane);
+                return;
+            }
+        }
+        int lo[9], hi[9], pb = 0, own = 0;
+        tra_rows<kBatch>(P, pi, active, i, lo, hi, &pb, &own);
+        bool lng = false;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) lng |= active && hi[r] - lo[r] > kMaskRowMax;
+        const bool any_long = __ballot(lng) != 0;
+        if (kLean) {
+            if (__ballot(active && own > kFourChainCell) != 0 || (P.use_masks && any_long)) {
+                defer_tile(P, it, lane);
+                return;
+            }
+            if (lane == 0) P.tile_defer[it] = 0;
+        }
+        const bool four = !kLean && own > kFourChainCell;
+        float s = 0.f;
+        bool long_row = false;
+        int pbase = -1;
+        bool pool = false;
+        {
+#pragma unroll
+            for (int r = 0; r < 9; ++r) {
+                rlo[r * 64 + lane] = lo[r];
+                rhi[r * 64 + lane] = hi[r];
+            }
+            // a tile with a row longer than one mask word keeps its masks in
+            // the pool: rows for an upper bound of its staging rounds (each
+            // row's rounds advance by at least kStage over the wave's span)
+            int pool_rounds = 0;
+            if (!kLean && P.use_masks && any_long) {
+                pool_rounds = tra_round_bound<kStage>(lo, hi);
+                pbase = pool_take(P, pool_rounds * (kStage / 32), min(P.sms_pool_rows, kPoolBaseMax),
+                                  lane);
+            }
+            pool = !kLean && pbase >= 0;
+            int round = 0;
+            // per-row partial sums (rejected candidates add exact zeros, so
+            // pruning cannot change the result) and per-row accept masks
+            RowSum rs = row_sum_zero();
+            uint32_t mk = 0;
+            int done = 0;   // rows at order positions [0, done) are finished
+            auto finish_rows = [&](int upto) __attribute__((always_inline)) {
+                for (; done < upto; ++done) {
+                    s += row_sum_value(rs, four);
+                    rs = row_sum_zero();
+                    if (P.use_masks && !pool) {
+                        const int r = row_of(done);
+                        const int cnt = rhi[r * 64 + lane] - rlo[r * 64 + lane];
+                        long_row |= cnt > kMaskRowMax;
+                        if (active)
+                            P.nbmask[(size_t)r * P.mask_stride + idx] =
+                                cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
+                    }
+                    mk = 0;
+                }
+            };
+            float4 v0 = pi, v1 = pi, v2 = pi, v3 = pi;
+            int q = -1, cur = kNone, lo_r = 0, hi_r = 0;
+            next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
+            if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
+            while (cur != kNone) {
+                const int qq = q, base = cur, lo_c = lo_r, hi_c = hi_r;
+                store_round<kStage>(stg, lane, v0, v1, v2, v3);
+                wave_sync_lds();
+                next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
+                if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
+                finish_rows(qq);
+                const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
+                const int cnt = je - jb;
+                if (pool) {
+                    // the same candidates in the same order, 32 window
+                    // positions per mask word
+#pragma unroll
+                    for (int c = 0; c < kStage / 32; ++c) {
+                        const int cb = max(jb, base + 32 * c), ce = min(je, base + 32 * c + 32);
+                        const int nc = ce - cb;
+                        uint32_t w = 0;
+                        if (nc > 0) {
+                            density_row(pi, stg + (cb - base), nc, cb - pb, four, pt, rs, w);
+                            w = finish_mask(w, nc) << (cb - base - 32 * c);
+                        }
+                        if (round < pool_rounds)   // always, by the bound
+                            P.sms_pool[(size_t)(pbase + round * (kStage / 32) + c) * 64 + lane] = w;
+                    }
+                    ++round;
+                } else if (cnt > 0) {
+                    density_row(pi, stg + (jb - base), cnt, jb - pb, four, pt, rs, mk);
+                }
+                wave_sync_lds();
+            }
+            finish_rows(9);
+        }
+        if (active) {
+            density_store(P, i, s);
+            if (P.use_masks) {
+                bool fits_plan = true;
+#pragma unroll
+                for (int r = 0; r < 9; ++r) {
+                    const int len = hi[r] - lo[r], off = len > 0 ? lo[r] - i : 0;
+                    fits_plan &= len <= 255 && off >= -(1 << 23) && off < (1 << 23);
+                    P.nbmask[(size_t)(10 + r) * P.mask_stride + idx] =
+                        ((uint32_t)off & 0xffffffu) | ((uint32_t)len << 24);
+                }
+                P.nbmask[(size_t)9 * P.mask_stride + idx] =
+                    pool ? (kFlagPool | ((uint32_t)pbase << 3) | (fits_plan ? kFlagPlan : 0u))
+                         : ((long_row ? kFlagLongRow : 0u) | (fits_plan ? kFlagPlan : 0u));
+            }
+        }
+}
+
+// The lean density pass: TRA tiles without four-chain lanes or pool rounds.
+template <bool kBatch, int kStage>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(kStage > 128 ? 4 : 6))) void k_density_lean(PhysicsArgs P) {
+    __shared__ float4 s_lds[kNT / 64][DensLds<kStage>::kWords];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float4 *wl = s_lds[wave];
+    const int tra = P.dc[kCntTra];
+    const ItemMap m = item_map((tra + 63) >> 6);
+    const PairTest pt = pair_test(P.c.kernel_2);
+    for (int it = m.first; it < m.end; it += m.stride) {
+        wave_sync_lds();   // the previous item's LDS reads precede this item's writes
+        density_tile<kBatch, kStage, true>(P, it, tra, lane, wl, pt);
+    }
+}
+
 // kLegacy: SMS items use the legacy staging (P.sms_legacy), a variant of its
 // own so that the default kernels carry none of its registers
 template <bool kBatch, int kStage, bool kLegacy>
@@ -689,124 +854,18 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     __shared__ float4 s_lds[kNT / 64][DensLds<kStage>::kWords];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float4 *wl = s_lds[wave];
-    // round walk: row bounds (ints) then the staging round; SMS: 64 entries
-    int *rlo = reinterpret_cast<int *>(wl), *rhi = rlo + 9 * 64;
-    float4 *stg = wl + 2 * 9 * 64 / 4;
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
-    const int tra_tiles = (tra + 63) >> 6;
+    // TRA tiles only when the lean pass left some in this arrange epoch
+    const int tra_tiles = P.dc[kCntAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const PairTest pt = pair_test(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
         if (it < tra_tiles) {
-            // all lanes stay in the wave (the row walk is cooperative);
-            // lanes past the TRA range carry empty rows
-            const int idx = it * 64 + lane;
-            const bool active = idx < tra;
-            const int i = P.compact[active ? idx : tra - 1];
-            const float4 pi = P.pos[i];
-            int lo[9], hi[9], pb = 0, own = 0;
-            tra_rows<kBatch>(P, pi, active, i, lo, hi, &pb, &own);
-            const bool four = own > kFourChainCell;
-            bool lng = false;
-#pragma unroll
-            for (int r = 0; r < 9; ++r) lng |= active && hi[r] - lo[r] > kMaskRowMax;
-            const bool any_long = __ballot(lng) != 0;
-            float s = 0.f;
-            bool long_row = false;
-            int pbase = -1;
-            bool pool = false;
-            {
-#pragma unroll
-                for (int r = 0; r < 9; ++r) {
-                    rlo[r * 64 + lane] = lo[r];
-                    rhi[r * 64 + lane] = hi[r];
-                }
-                // a tile with a row longer than one mask word keeps its masks in
-                // the pool: rows for an upper bound of its staging rounds (each
-                // row's rounds advance by at least kStage over the wave's span)
-                int pool_rounds = 0;
-                if (P.use_masks && any_long) {
-                    pool_rounds = tra_round_bound<kStage>(lo, hi);
-                    pbase = pool_take(P, pool_rounds * (kStage / 32), min(P.sms_pool_rows, kPoolBaseMax),
-                                      lane);
-                }
-                pool = pbase >= 0;
-                int round = 0;
-                // per-row partial sums (rejected candidates add exact zeros, so
-                // pruning cannot change the result) and per-row accept masks
-                RowSum rs = row_sum_zero();
-                uint32_t mk = 0;
-                int done = 0;   // rows at order positions [0, done) are finished
-                auto finish_rows = [&](int upto) __attribute__((always_inline)) {
-                    for (; done < upto; ++done) {
-                        s += row_sum_value(rs, four);
-                        rs = row_sum_zero();
-                        if (P.use_masks && !pool) {
-                            const int r = row_of(done);
-                            const int cnt = rhi[r * 64 + lane] - rlo[r * 64 + lane];
-                            long_row |= cnt > kMaskRowMax;
-                            if (active)
-                                P.nbmask[(size_t)r * P.mask_stride + idx] =
-                                    cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
-                        }
-                        mk = 0;
-                    }
-                };
-                float4 v0 = pi, v1 = pi, v2 = pi, v3 = pi;
-                int q = -1, cur = kNone, lo_r = 0, hi_r = 0;
-                next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
-                if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
-                while (cur != kNone) {
-                    const int qq = q, base = cur, lo_c = lo_r, hi_c = hi_r;
-                    store_round<kStage>(stg, lane, v0, v1, v2, v3);
-                    wave_sync_lds();
-                    next_round<kStage>(q, cur, lo_r, hi_r, rlo, rhi, lane);
-                    if (cur != kNone) load_round<kStage>(P.pos, cur, P.n, lane, v0, v1, v2, v3);
-                    finish_rows(qq);
-                    const int jb = max(lo_c, base), je = min(hi_c, base + kStage);
-                    const int cnt = je - jb;
-                    if (pool) {
-                        // the same candidates in the same order, 32 window
-                        // positions per mask word
-#pragma unroll
-                        for (int c = 0; c < kStage / 32; ++c) {
-                            const int cb = max(jb, base + 32 * c), ce = min(je, base + 32 * c + 32);
-                            const int nc = ce - cb;
-                            uint32_t w = 0;
-                            if (nc > 0) {
-                                density_row(pi, stg + (cb - base), nc, cb - pb, four, pt, rs, w);
-                                w = finish_mask(w, nc) << (cb - base - 32 * c);
-                            }
-                            if (round < pool_rounds)   // always, by the bound
-                                P.sms_pool[(size_t)(pbase + round * (kStage / 32) + c) * 64 + lane] = w;
-                        }
-                        ++round;
-                    } else if (cnt > 0) {
-                        density_row(pi, stg + (jb - base), cnt, jb - pb, four, pt, rs, mk);
-                    }
-                    wave_sync_lds();
-                }
-                finish_rows(9);
-            }
-            if (active) {
-                density_store(P, i, s);
-                if (P.use_masks) {
-                    bool fits_plan = true;
-#pragma unroll
-                    for (int r = 0; r < 9; ++r) {
-                        const int len = hi[r] - lo[r], off = len > 0 ? lo[r] - i : 0;
-                        fits_plan &= len <= 255 && off >= -(1 << 23) && off < (1 << 23);
-                        P.nbmask[(size_t)(10 + r) * P.mask_stride + idx] =
-                            ((uint32_t)off & 0xffffffu) | ((uint32_t)len << 24);
-                    }
-                    P.nbmask[(size_t)9 * P.mask_stride + idx] =
-                        pool ? (kFlagPool | ((uint32_t)pbase << 3) | (fits_plan ? kFlagPlan : 0u))
-                             : ((long_row ? kFlagLongRow : 0u) | (fits_plan ? kFlagPlan : 0u));
-                }
-            }
+            // the tiles the lean pass left (tile_defer, written by it)
+            if (P.tile_defer[it]) density_tile<kBatch, kStage, false>(P, it, tra, lane, wl, pt);
         } else {
             const int item = it - tra_tiles;
             const SmsItem sm = sms_item<kBatch>(P, item << 1, T, lane);
@@ -1567,6 +1626,7 @@ __global__ void k_checksum_final(const double *partial, int nblocks, double *out
 
 template <bool B, int S>
 void launch_density_t(const PhysicsArgs &p, hipStream_t s) {
+    hipLaunchKernelGGL((k_density_lean<B, S>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     if (p.sms_legacy)
         hipLaunchKernelGGL((k_density<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     else
