@@ -1141,6 +1141,118 @@ __device__ __forceinline__ uint32_t test_row(const float4 &pi, const float4 *q, 
     return finish_mask(mask, cnt);
 }
 
+// The lean force pass.  It takes the TRA tiles where every lane has the
+// density pass's row plan and accepted-pair masks (no row longer than one
+// mask word, no pool rounds) and every mirrored row pair fits kForceCap
+// entries: the staged-row path alone, which fits 80 registers, in
+// kForceLeanBufs group buffers.  Every other tile is flagged in tile_defer
+// for the general pass (k_force); both give the same bits.
+#ifndef FORCE_LEAN_BUFS
+#define FORCE_LEAN_BUFS 1
+#endif
+constexpr int kForceLeanBufs = FORCE_LEAN_BUFS;
+
+__device__ __forceinline__ void defer_force_tile(const PhysicsArgs &P, int it, int lane) {
+    if (lane == 0) {
+        P.tile_defer[it] = 1;
+        const int ep = P.dc[kCntEpoch];
+        if (*reinterpret_cast<const volatile int *>(&P.dc[kCntForceAnyDefer]) != ep)
+            P.dcw[kCntForceAnyDefer] = ep;
+    }
+}
+
+template <bool kBatch>
+__device__ __forceinline__ void force_tile_lean(const PhysicsArgs &P, int it, int tra, int lane,
+                                                float4 *wl, HH hh) {
+    const int idx = it * 64 + lane;
+    const bool active = idx < tra;
+    const int i = P.compact[active ? idx : tra - 1];
+    const uint32_t flag = P.use_masks && active ? P.nbmask[(size_t)9 * P.mask_stride + idx] : 0u;
+    const bool planned = (flag & kFlagPlan) && !(flag & (kFlagLongRow | kFlagPool));
+    if (!P.use_masks || __ballot(active && !planned) != 0) {
+        defer_force_tile(P, it, lane);
+        return;
+    }
+    const float4 pi = P.pos[i];
+    const float4 ei = P.ev[i];
+    int lo[9], hi[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const uint32_t w = active ? P.nbmask[(size_t)(10 + r) * P.mask_stride + idx] : 0u;
+        lo[r] = active ? i + ((int)(w << 8) >> 8) : 0;
+        hi[r] = active ? lo[r] + (int)(w >> 24) : 0;
+    }
+    int L[9], S[9];
+    row_spans(lo, hi, L, S);
+    bool fits = true;
+#pragma unroll
+    for (int g = 0; g < 5; ++g)
+        fits = fits && S[row_of(2 * g)] + (g < 4 ? S[row_of(2 * g + 1)] : 0) <= kForceCap;
+    if (!fits) {
+        defer_force_tile(P, it, lane);
+        return;
+    }
+    if (lane == 0) P.tile_defer[it] = 0;
+    ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t nma = 0u, nmb = 0u;
+    auto issue = [&](int g) __attribute__((always_inline)) {
+        float4 *bp = wl + (g % kForceLeanBufs) * 2 * kForceCap, *be = bp + kForceCap;
+        const int ra = row_of(2 * g), rb = g < 4 ? row_of(2 * g + 1) : 0;
+        const int sb = g < 4 ? S[rb] : 0;
+        dma_span(P.pos, L[ra], S[ra], bp, lane);
+        dma_span(P.ev, L[ra], S[ra], be, lane);
+        dma_span(P.pos, L[rb], sb, bp + S[ra], lane);
+        dma_span(P.ev, L[rb], sb, be + S[ra], lane);
+        nma = active ? P.nbmask[(size_t)ra * P.mask_stride + idx] : 0u;
+        nmb = active && g < 4 ? P.nbmask[(size_t)rb * P.mask_stride + idx] : 0u;
+    };
+    if (kForceLeanBufs > 1) issue(0);
+#pragma unroll
+    for (int g = 0; g < 5; ++g) {
+        if (kForceLeanBufs == 1) {
+            if (g > 0) wave_sync_lds();   // the previous group's reads precede these writes
+            issue(g);
+        }
+        wait_dma();
+        uint32_t ma = nma, mb = nmb;
+        if (kForceLeanBufs > 1 && g + 1 < 5) issue(g + 1);
+        const float4 *bp = wl + (g % kForceLeanBufs) * 2 * kForceCap, *be = bp + kForceCap;
+        const int ra = row_of(2 * g), rb = g < 4 ? row_of(2 * g + 1) : 0;
+        const int oa = lo[ra] - L[ra];
+        const int ob = g < 4 ? S[ra] + lo[rb] - L[rb] : 0;
+        while (ma | mb) {
+            const bool ua = ma != 0;
+            const int j = (ua ? oa : ob) + __builtin_ctz(ua ? ma : mb);
+            if (ua)
+                ma &= ma - 1u;
+            else
+                mb &= mb - 1u;
+            force_pair(pi, ei, bp[j], be[j], P.c, hh, A);
+        }
+    }
+    if (active) force_store<kBatch>(P, i, pi, ei, A);
+}
+
+template <bool kBatch>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(kForceLeanBufs > 1 ? 4 : 6)))
+void k_force_lean(PhysicsArgs P) {
+    __shared__ float4 s_lds[kNT / 64][kForceLeanBufs * 2 * kForceCap];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float4 *wl = s_lds[wave];
+    const int tra = P.dc[kCntTra];
+    // integrated here: the state goes back in sorted order (see k_integrate)
+    if (P.fuse_integrate && blockIdx.x == 0 && threadIdx.x == 0) {
+        P.dcw[kCntOrder] = 1;
+        P.dcw[kCntKeysReady] = 1;
+    }
+    const ItemMap m = item_map((tra + 63) >> 6);
+    const HH hh = exact_h2(P.c.kernel_h);
+    for (int it = m.first; it < m.end; it += m.stride) {
+        wave_sync_lds();   // the previous item's LDS reads precede this item's writes
+        force_tile_lean<kBatch>(P, it, tra, lane, wl, hh);
+    }
+}
+
 template <bool kBatch, int kStage, bool kLegacy>
 __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     __shared__ float4 s_lds[kNT / 64][ForceLds<kStage>::kWords];
@@ -1157,13 +1269,15 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
         P.dcw[kCntOrder] = 1;
         P.dcw[kCntKeysReady] = 1;
     }
-    const int tra_tiles = (tra + 63) >> 6;
+    // TRA tiles only when the lean pass left some in this arrange epoch
+    const int tra_tiles = P.dc[kCntForceAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const HH hh = exact_h2(P.c.kernel_h);
     const PairTest pt = pair_test(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
+        if (it < tra_tiles && !P.tile_defer[it]) continue;   // the lean pass took it
         if (it < tra_tiles) {
             const int idx = it * 64 + lane;
             const bool active = idx < tra;
@@ -1635,6 +1749,7 @@ void launch_density_t(const PhysicsArgs &p, hipStream_t s) {
 
 template <bool B, int S>
 void launch_force_t(const PhysicsArgs &p, hipStream_t s) {
+    hipLaunchKernelGGL((k_force_lean<B>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     if (p.sms_legacy)
         hipLaunchKernelGGL((k_force<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     else
