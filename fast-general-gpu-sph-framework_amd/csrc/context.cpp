@@ -287,6 +287,7 @@ PhysicsArgs physics_args(gsph_ctx *ctx) {
     p.row_prune = ctx->opt.row_prune;
     p.sms_local_merge = ctx->opt.sms_local_merge;
     p.sms_legacy = ctx->opt.sms_legacy_staging;
+    p.lean = ctx->opt.lean_passes;
     p.scf = ctx->opt.same_cell_pair_force ? ctx->scf : nullptr;
     p.scf_stride = ctx->cap_n;
     p.nbmask = ctx->nbmask;
@@ -620,6 +621,8 @@ int check_options(gsph_ctx *ctx, gsph_options &o) {
         return fail(ctx, GSPH_ERR_INVALID_ARG, "sms_legacy_staging must be 0 or 1");
     if (o.same_cell_pair_force != 0 && o.same_cell_pair_force != 1)
         return fail(ctx, GSPH_ERR_INVALID_ARG, "same_cell_pair_force must be 0 or 1");
+    if (o.lean_passes != 0 && o.lean_passes != 1)
+        return fail(ctx, GSPH_ERR_INVALID_ARG, "lean_passes must be 0 or 1");
     return GSPH_OK;
 }
 
@@ -1488,6 +1491,11 @@ int gsph_autotune(gsph_ctx *ctx, int trial_steps, int tune_split, gsph_options *
     {   // the legacy SMS staging on / off (same results; no effect without SMS tasks)
         gsph_options o = best;
         o.sms_legacy_staging ^= 1;
+        trial(o);
+    }
+    {   // the lean density/force passes on / off (same results)
+        gsph_options o = best;
+        o.lean_passes ^= 1;
         trial(o);
     }
     (void)hipFree(pos0);

@@ -119,6 +119,7 @@ struct PhysicsArgs {
     uint32_t *nbmask;
     int mask_stride;
     int use_masks;
+    int lean;              // 1: the lean density/force passes run first (option lean_passes)
     int stage_depth;       // LDS staging depth of TRA rows: 64, 128 or 256
     // accepted-pair masks of SMS items: a pool of rows of 64 words (one per
     // lane); density takes W rows per item (W = 32-candidate sub-batches of

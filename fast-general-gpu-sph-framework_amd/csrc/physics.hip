@@ -856,8 +856,9 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     float4 *wl = s_lds[wave];
     const int tra = P.dc[kCntTra];
     const int T = P.dc[kCntT];
-    // TRA tiles only when the lean pass left some in this arrange epoch
-    const int tra_tiles = P.dc[kCntAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
+    // without the lean pass every TRA tile; with it, only when it left some
+    // in this arrange epoch
+    const int tra_tiles = !P.lean || P.dc[kCntAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const PairTest pt = pair_test(P.c.kernel_2);
@@ -865,7 +866,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
         if (it < tra_tiles) {
             // the tiles the lean pass left (tile_defer, written by it)
-            if (P.tile_defer[it]) density_tile<kBatch, kStage, false>(P, it, tra, lane, wl, pt);
+            if (!P.lean || P.tile_defer[it]) density_tile<kBatch, kStage, false>(P, it, tra, lane, wl, pt);
         } else {
             const int item = it - tra_tiles;
             const SmsItem sm = sms_item<kBatch>(P, item << 1, T, lane);
@@ -1269,15 +1270,16 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
         P.dcw[kCntOrder] = 1;
         P.dcw[kCntKeysReady] = 1;
     }
-    // TRA tiles only when the lean pass left some in this arrange epoch
-    const int tra_tiles = P.dc[kCntForceAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
+    // without the lean pass every TRA tile; with it, only when it left some
+    // in this arrange epoch
+    const int tra_tiles = !P.lean || P.dc[kCntForceAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const HH hh = exact_h2(P.c.kernel_h);
     const PairTest pt = pair_test(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
-        if (it < tra_tiles && !P.tile_defer[it]) continue;   // the lean pass took it
+        if (it < tra_tiles && P.lean && !P.tile_defer[it]) continue;   // the lean pass took it
         if (it < tra_tiles) {
             const int idx = it * 64 + lane;
             const bool active = idx < tra;
@@ -1740,7 +1742,7 @@ __global__ void k_checksum_final(const double *partial, int nblocks, double *out
 
 template <bool B, int S>
 void launch_density_t(const PhysicsArgs &p, hipStream_t s) {
-    hipLaunchKernelGGL((k_density_lean<B, S>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    if (p.lean) hipLaunchKernelGGL((k_density_lean<B, S>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     if (p.sms_legacy)
         hipLaunchKernelGGL((k_density<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     else
@@ -1749,7 +1751,7 @@ void launch_density_t(const PhysicsArgs &p, hipStream_t s) {
 
 template <bool B, int S>
 void launch_force_t(const PhysicsArgs &p, hipStream_t s) {
-    hipLaunchKernelGGL((k_force_lean<B>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    if (p.lean) hipLaunchKernelGGL((k_force_lean<B>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     if (p.sms_legacy)
         hipLaunchKernelGGL((k_force<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     else

@@ -225,6 +225,7 @@ void gsph_options_default(gsph_options *out) {
     out->sms_local_merge = 1;
     out->sms_legacy_staging = 0;
     out->same_cell_pair_force = 0;
+    out->lean_passes = 1;
 }
 
 int gsph_derive_constants(const gsph_params *p, gsph_constants *c) {

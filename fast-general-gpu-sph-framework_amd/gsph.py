@@ -75,6 +75,7 @@ class Options(C.Structure):
         ("blocks_per_cu", C.c_int), ("pair_masks", C.c_int), ("row_prune", C.c_int),
         ("stage_depth", C.c_int), ("sms_mask_bytes", C.c_int), ("sms_local_merge", C.c_int),
         ("sms_legacy_staging", C.c_int), ("same_cell_pair_force", C.c_int),
+        ("lean_passes", C.c_int),
     ]
 
 

@@ -428,6 +428,28 @@ def test_launch_shape_changes_nothing(gsph, po, name, opts, depth, bpc):
         assert np.array_equal(out[0][k], out[1][k])
 
 
+@pytest.mark.parametrize("name,opts", [("cfg0", {}), ("mixed", {}), ("dense", ALL_TRA),
+                                       ("sparse_random", {}), ("mixed", {"pair_masks": 0})])
+def test_lean_passes_change_nothing(gsph, po, name, opts):
+    """The lean density/force passes (the TRA tiles without four-chain sums or
+    pool rounds, at higher occupancy) and the general passes alone give the
+    same bits (the physics tests, which check the FP64 bar, run with the lean
+    passes on, the default): cfg0 and sparse_random are taken by the lean
+    passes, the dense all-TRA tiles (four chains, pool rounds) and the SMS
+    items are left to the general ones, mixed has both, and without pair
+    masks the lean force pass leaves every tile."""
+    op, pos = scene(po, name)
+    banner(f"lean passes {name} {opts}")
+    out = []
+    for lean in (1, 0):
+        s = make_system(gsph, op, pos, lean_passes=lean, **opts)
+        s.step(2)
+        out.append(s.read_state())
+        s.close()
+    for k in range(4):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
 @pytest.mark.parametrize("name", ["mixed", "dense64", "odd_tasks"])
 def test_sms_local_merge_changes_no_bits(gsph, po, name):
     """Shared staging of an isSame pair's rows (the reference's SMS local
