@@ -417,12 +417,22 @@ def main():
     step_gbs = alg["step"] / (ms_per_step * 1e-3) / 1e9
 
     def single_scene(table, stage):
-        """Entry of the single-scene instance of a stage's kernel (its name
-        starts with k_<stage><false), any staging-depth argument)."""
-        for k, v in table.items():
-            if k.startswith(f"k_{stage}<false"):
-                return v
-        return {}
+        """A stage's single-scene kernels summed per launch: its lean pass
+        (k_<stage>_lean<false...>) and its general pass (k_<stage><false...>),
+        any staging-depth argument; the lanes-active share recomputed from
+        the summed counters."""
+        parts = [v for k, v in table.items()
+                 if k.startswith(f"k_{stage}<false") or k.startswith(f"k_{stage}_lean<false")]
+        if not parts:
+            return {}
+        out = {}
+        for key in ("bytes", "SQ_THREAD_CYCLES_VALU", "SQ_INSTS_VALU"):
+            vals = [p[key] for p in parts if key in p]
+            if vals:
+                out[key] = sum(vals)
+        if out.get("SQ_INSTS_VALU"):
+            out["valu_util_pct"] = 100.0 * out["SQ_THREAD_CYCLES_VALU"] / (64.0 * out["SQ_INSTS_VALU"])
+        return out
 
     traffic = profile_block(TRAFFIC_PROFILE).get("traffic", {})
     valu = profile_block(VALU_PROFILE).get("counters", {})
