@@ -37,8 +37,9 @@ enum CounterSlot {
     kCntOrder = 13,     // 1: the slot order is the order of hash[] (set by integrate)
     kCntEpoch = 14,     // arranges so far (tags the sort's look-back words)
     kCntKeysReady = 15, // 1: next_keys holds the micro ids of the slot order's state
-    kCntForceAnyDefer = 16,  // the same for the lean force pass
-    kCntSlots = 17,
+    kCntForceAnyDefer = 16,  // the same for the first lean force pass
+    kCntForceAnyDefer2 = 17, // and for the second (windowed) one
+    kCntSlots = 18,
 };
 
 // Geometry of one scene inside the (possibly batched) virtual grid.  In the

@@ -1142,36 +1142,45 @@ __device__ __forceinline__ uint32_t test_row(const float4 &pi, const float4 *q, 
     return finish_mask(mask, cnt);
 }
 
-// The lean force pass.  It takes the TRA tiles where every lane has the
-// density pass's row plan and accepted-pair masks (no row longer than one
-// mask word, no pool rounds) and every mirrored row pair fits kForceCap
-// entries: the staged-row path alone, which fits 80 registers, in
-// kForceLeanBufs group buffers.  Every other tile is flagged in tile_defer
-// for the general pass (k_force); both give the same bits.
-#ifndef FORCE_LEAN_BUFS
-#define FORCE_LEAN_BUFS 1
-#endif
-constexpr int kForceLeanBufs = FORCE_LEAN_BUFS;
-
-__device__ __forceinline__ void defer_force_tile(const PhysicsArgs &P, int it, int lane) {
+// The lean force passes take the TRA tiles where every lane has the density
+// pass's row plan and accepted-pair masks (no row longer than one mask word,
+// no pool rounds): the staged-row path alone, in one group buffer of
+// kForceCap entries (5 KB of LDS).  The first (kWin = false) takes the tiles
+// whose mirrored row pairs each fit the buffer, in 72 registers, seven waves
+// per SIMD; the second (kWin = true) the planned tiles the first left (tiles
+// across cell rows of small scenes), each pair streamed through the buffer in
+// windows.  Every other tile is flagged in tile_defer for the general pass
+// (k_force); all give the same bits.  slot: the counter that records, once
+// per arrange epoch, that a pass left a tile (the windowed pass, always the
+// last lean one, uses the one the general pass reads).
+__device__ __forceinline__ void defer_force_tile(const PhysicsArgs &P, int it, int lane, int slot) {
     if (lane == 0) {
         P.tile_defer[it] = 1;
         const int ep = P.dc[kCntEpoch];
-        if (*reinterpret_cast<const volatile int *>(&P.dc[kCntForceAnyDefer]) != ep)
-            P.dcw[kCntForceAnyDefer] = ep;
+        if (*reinterpret_cast<const volatile int *>(&P.dc[slot]) != ep) P.dcw[slot] = ep;
     }
 }
 
-template <bool kBatch>
+// Bits k of mask m (candidate lo + k, at position o + k of a row pair's
+// staged span) whose position lies in the window [ws, we).
+__device__ __forceinline__ uint32_t window_bits(uint32_t m, int o, int ws, int we) {
+    const int k0 = max(ws - o, 0), k1 = min(we - o, 32);
+    if (k1 <= k0) return 0u;
+    const uint32_t upto = k1 >= 32 ? 0xffffffffu : (1u << k1) - 1u;
+    return m & upto & ~((1u << k0) - 1u);
+}
+
+template <bool kBatch, bool kWin>
 __device__ __forceinline__ void force_tile_lean(const PhysicsArgs &P, int it, int tra, int lane,
                                                 float4 *wl, HH hh) {
+    const int slot = kWin ? kCntForceAnyDefer2 : kCntForceAnyDefer;
     const int idx = it * 64 + lane;
     const bool active = idx < tra;
     const int i = P.compact[active ? idx : tra - 1];
     const uint32_t flag = P.use_masks && active ? P.nbmask[(size_t)9 * P.mask_stride + idx] : 0u;
     const bool planned = (flag & kFlagPlan) && !(flag & (kFlagLongRow | kFlagPool));
     if (!P.use_masks || __ballot(active && !planned) != 0) {
-        defer_force_tile(P, it, lane);
+        defer_force_tile(P, it, lane, slot);
         return;
     }
     const float4 pi = P.pos[i];
@@ -1185,59 +1194,68 @@ __device__ __forceinline__ void force_tile_lean(const PhysicsArgs &P, int it, in
     }
     int L[9], S[9];
     row_spans(lo, hi, L, S);
-    bool fits = true;
+    if (!kWin) {
+        bool fits = true;
 #pragma unroll
-    for (int g = 0; g < 5; ++g)
-        fits = fits && S[row_of(2 * g)] + (g < 4 ? S[row_of(2 * g + 1)] : 0) <= kForceCap;
-    if (!fits) {
-        defer_force_tile(P, it, lane);
-        return;
+        for (int g = 0; g < 5; ++g)
+            fits = fits && S[row_of(2 * g)] + (g < 4 ? S[row_of(2 * g + 1)] : 0) <= kForceCap;
+        if (!fits) {
+            defer_force_tile(P, it, lane, slot);
+            return;
+        }
     }
     if (lane == 0) P.tile_defer[it] = 0;
     ForceAcc A = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t nma = 0u, nmb = 0u;
-    auto issue = [&](int g) __attribute__((always_inline)) {
-        float4 *bp = wl + (g % kForceLeanBufs) * 2 * kForceCap, *be = bp + kForceCap;
-        const int ra = row_of(2 * g), rb = g < 4 ? row_of(2 * g + 1) : 0;
-        const int sb = g < 4 ? S[rb] : 0;
-        dma_span(P.pos, L[ra], S[ra], bp, lane);
-        dma_span(P.ev, L[ra], S[ra], be, lane);
-        dma_span(P.pos, L[rb], sb, bp + S[ra], lane);
-        dma_span(P.ev, L[rb], sb, be + S[ra], lane);
-        nma = active ? P.nbmask[(size_t)ra * P.mask_stride + idx] : 0u;
-        nmb = active && g < 4 ? P.nbmask[(size_t)rb * P.mask_stride + idx] : 0u;
-    };
-    if (kForceLeanBufs > 1) issue(0);
+    float4 *bp = wl, *be = wl + kForceCap;
 #pragma unroll
     for (int g = 0; g < 5; ++g) {
-        if (kForceLeanBufs == 1) {
-            if (g > 0) wave_sync_lds();   // the previous group's reads precede these writes
-            issue(g);
-        }
-        wait_dma();
-        uint32_t ma = nma, mb = nmb;
-        if (kForceLeanBufs > 1 && g + 1 < 5) issue(g + 1);
-        const float4 *bp = wl + (g % kForceLeanBufs) * 2 * kForceCap, *be = bp + kForceCap;
         const int ra = row_of(2 * g), rb = g < 4 ? row_of(2 * g + 1) : 0;
+        const int sa = S[ra], sb = g < 4 ? S[rb] : 0;
+        // the pair's span: row a at [0, sa), row b at [sa, sa + sb)
         const int oa = lo[ra] - L[ra];
-        const int ob = g < 4 ? S[ra] + lo[rb] - L[rb] : 0;
-        while (ma | mb) {
-            const bool ua = ma != 0;
-            const int j = (ua ? oa : ob) + __builtin_ctz(ua ? ma : mb);
-            if (ua)
-                ma &= ma - 1u;
-            else
-                mb &= mb - 1u;
-            force_pair(pi, ei, bp[j], be[j], P.c, hh, A);
+        const int ob = sa + lo[rb] - L[rb];
+        const uint32_t fa = active ? P.nbmask[(size_t)ra * P.mask_stride + idx] : 0u;
+        const uint32_t fb = active && g < 4 ? P.nbmask[(size_t)rb * P.mask_stride + idx] : 0u;
+        // one window when the pair fits (always, in the first pass)
+        const int span = kWin ? sa + sb : 1;
+        for (int ws = 0; ws < span; ws += kForceCap) {   // wave-uniform
+            const int we = kWin ? min(ws + kForceCap, sa + sb) : sa + sb;
+            if (kWin || g > 0) wave_sync_lds();   // earlier reads of the buffer precede these writes
+            const int a1 = min(we, sa), b0 = max(ws, sa);
+            if (a1 > ws) {
+                dma_span(P.pos, L[ra] + ws, a1 - ws, bp, lane);
+                dma_span(P.ev, L[ra] + ws, a1 - ws, be, lane);
+            }
+            if (we > b0) {
+                dma_span(P.pos, L[rb] + b0 - sa, we - b0, bp + (b0 - ws), lane);
+                dma_span(P.ev, L[rb] + b0 - sa, we - b0, be + (b0 - ws), lane);
+            }
+            wait_dma();
+            // row a's pairs, then row b's (all of row a's positions come first)
+            uint32_t ma = kWin ? window_bits(fa, oa, ws, we) : fa;
+            uint32_t mb = kWin ? window_bits(fb, ob, ws, we) : fb;
+            while (ma | mb) {
+                const bool ua = ma != 0;
+                const int j = (ua ? oa : ob) - ws + __builtin_ctz(ua ? ma : mb);
+                if (ua)
+                    ma &= ma - 1u;
+                else
+                    mb &= mb - 1u;
+                force_pair(pi, ei, bp[j], be[j], P.c, hh, A);
+            }
         }
     }
     if (active) force_store<kBatch>(P, i, pi, ei, A);
 }
 
-template <bool kBatch>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(kForceLeanBufs > 1 ? 4 : 6)))
+// kAll: the pass takes every TRA tile (the first pass); else only the tiles
+// the first pass left.  Plain contexts run the fit-only pass and then the
+// windowed one; batched contexts, whose small scenes put many tiles across
+// cell rows, run the windowed pass over every tile.
+template <bool kBatch, bool kWin, bool kAll>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(kWin ? 6 : 7)))
 void k_force_lean(PhysicsArgs P) {
-    __shared__ float4 s_lds[kNT / 64][kForceLeanBufs * 2 * kForceCap];
+    __shared__ float4 s_lds[kNT / 64][2 * kForceCap];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float4 *wl = s_lds[wave];
     const int tra = P.dc[kCntTra];
@@ -1246,11 +1264,14 @@ void k_force_lean(PhysicsArgs P) {
         P.dcw[kCntOrder] = 1;
         P.dcw[kCntKeysReady] = 1;
     }
-    const ItemMap m = item_map((tra + 63) >> 6);
+    // a second pass only in an epoch where the first left a tile
+    const int tiles = kAll || P.dc[kCntForceAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
+    const ItemMap m = item_map(tiles);
     const HH hh = exact_h2(P.c.kernel_h);
     for (int it = m.first; it < m.end; it += m.stride) {
+        if (!kAll && !P.tile_defer[it]) continue;   // the first pass took it
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
-        force_tile_lean<kBatch>(P, it, tra, lane, wl, hh);
+        force_tile_lean<kBatch, kWin>(P, it, tra, lane, wl, hh);
     }
 }
 
@@ -1272,7 +1293,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     }
     // without the lean pass every TRA tile; with it, only when it left some
     // in this arrange epoch
-    const int tra_tiles = !P.lean || P.dc[kCntForceAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
+    const int tra_tiles = !P.lean || P.dc[kCntForceAnyDefer2] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const HH hh = exact_h2(P.c.kernel_h);
@@ -1751,7 +1772,12 @@ void launch_density_t(const PhysicsArgs &p, hipStream_t s) {
 
 template <bool B, int S>
 void launch_force_t(const PhysicsArgs &p, hipStream_t s) {
-    if (p.lean) hipLaunchKernelGGL((k_force_lean<B>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    if (p.lean && B) {
+        hipLaunchKernelGGL((k_force_lean<B, true, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    } else if (p.lean) {
+        hipLaunchKernelGGL((k_force_lean<B, false, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+        hipLaunchKernelGGL((k_force_lean<B, true, false>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+    }
     if (p.sms_legacy)
         hipLaunchKernelGGL((k_force<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     else

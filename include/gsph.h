@@ -151,11 +151,11 @@ typedef struct gsph_options {
                              force_kernels.cu:9-131, CMakeLists.txt:92-100);
                              needs 40 B per particle of capacity */
     int lean_passes;      /* 1: density and force first run lean passes that
-                             take the TRA tiles needing no four-chain sums,
-                             no pool rounds and (force) whose row pairs fit
-                             one staging buffer, at higher occupancy; the
-                             general passes take the rest.  0: the general
-                             passes take every tile.  Same results */
+                             take the TRA tiles needing no four-chain sums
+                             and no pool rounds (force: with density's pair
+                             masks), at higher occupancy; the general passes
+                             take the rest.  0: the general passes take
+                             every tile.  Same results */
 } gsph_options;
 
 typedef struct gsph_ctx gsph_ctx;
