@@ -298,6 +298,11 @@ __device__ __forceinline__ uint32_t finish_mask(uint32_t shifted, int cnt) {
     return cnt > 0 ? __builtin_bitreverse32(shifted) >> (32 - cnt) : 0u;
 }
 
+// The accept mask of cnt candidates from density_pair's rejection bits.
+__device__ __forceinline__ uint32_t finish_rejected(uint32_t rej, int cnt) {
+    return finish_mask(~rej, cnt);
+}
+
 // One SMS work item: tasks t0 = 2q and t0 + 1, one per half-wave.  When both
 // are tasks of one cell (a judgeTask isSame pair) their rows are the same
 // cell rows, and the wave stages the union of their x-layer ranges once, 64
@@ -437,13 +442,17 @@ __device__ __forceinline__ int legacy_segments(const PhysicsArgs &P, const Scene
 // One candidate of the density sum: rs += (h^2 - r^2)^3 if the pair is
 // accepted (a rejected one adds an exact zero), and its acceptance bit is
 // shifted into mk.
+// One density pair: rs += t^3 for an accepted pair; rej collects one bit per
+// candidate (bits enter at the bottom), 1 for a rejected pair.  A rejected
+// pair's t is -0, which adds nothing (the sums start at +0 and never become
+// -0) and whose sign bit is the rejection bit, shifted in with one
+// v_alignbit; an accepted t is non-negative.
 __device__ __forceinline__ void density_pair(const float4 &a, const float4 &b, PairTest pt,
-                                             float &rs, uint32_t &mk) {
+                                             float &rs, uint32_t &rej) {
     const float t0 = kernel_t(a, b, pt.h2);
-    const bool ok = accept_t(t0, pt);
-    const float t = ok ? t0 : 0.f;
+    const float t = accept_t(t0, pt) ? t0 : -0.f;
     rs = fmaf(t * t, t, rs);
-    mk += mk + (ok ? 1u : 0u);
+    rej = __builtin_amdgcn_alignbit(rej, __float_as_uint(t), 31);
 }
 
 __device__ __forceinline__ void density_store(const PhysicsArgs &P, int i, float s) {
@@ -648,8 +657,9 @@ __device__ __forceinline__ void chains_rotate(float &a0, float &a1, float &a2, f
     }
 }
 
-// Candidates q[0, cnt) of a row, the first at place off of its scene; accept
-// mask mk (bits in candidate order enter at the bottom).
+// Candidates q[0, cnt) of a row, the first at place off of its scene;
+// rejection bits mk (in candidate order, entering at the bottom; see
+// density_pair).
 __device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, int cnt, int off,
                                             bool four, PairTest pt, RowSum &rs,
                                             uint32_t &mk) {
@@ -759,7 +769,7 @@ __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int t
             pool = !kLean && pbase >= 0;
             int round = 0;
             // per-row partial sums (rejected candidates add exact zeros, so
-            // pruning cannot change the result) and per-row accept masks
+            // pruning cannot change the result) and per-row rejection bits
             RowSum rs = row_sum_zero();
             uint32_t mk = 0;
             int done = 0;   // rows at order positions [0, done) are finished
@@ -773,7 +783,7 @@ __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int t
                         long_row |= cnt > kMaskRowMax;
                         if (active)
                             P.nbmask[(size_t)r * P.mask_stride + idx] =
-                                cnt > kMaskRowMax ? 0u : finish_mask(mk, cnt);
+                                cnt > kMaskRowMax ? 0u : finish_rejected(mk, cnt);
                     }
                     mk = 0;
                 }
@@ -801,7 +811,7 @@ __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int t
                         uint32_t w = 0;
                         if (nc > 0) {
                             density_row(pi, stg + (cb - base), nc, cb - pb, four, pt, rs, w);
-                            w = finish_mask(w, nc) << (cb - base - 32 * c);
+                            w = finish_rejected(w, nc) << (cb - base - 32 * c);
                         }
                         if (round < pool_rounds)   // always, by the bound
                             P.sms_pool[(size_t)(pbase + round * (kStage / 32) + c) * 64 + lane] = w;
@@ -928,7 +938,7 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
                             const int n = min(32, cnt - k0);
                             uint32_t mk = 0;
                             density_row(pi, lds + k0, n, b0 + k0 - g.p_begin, four, pt, rs, mk);
-                            if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_mask(mk, n);
+                            if (base >= 0) P.sms_pool[(size_t)(base + w) * 64 + lane] = finish_rejected(mk, n);
                         }
                         wave_sync_lds();
                     }
