@@ -568,9 +568,21 @@ __device__ __forceinline__ void next_round(int &s, int &cur, int &lo_r, int &hi_
     int nxt = kNone;
     // only lanes with candidates in the row steer the walk, so the rounds
     // depend on the live ranges alone (density and force must walk the same
-    // rounds: the pool masks are indexed by round)
-    if (s >= 0)
-        nxt = wave_min(lo_r < hi_r && hi_r > cur + kStage ? max(lo_r, cur + kStage) : kNone);
+    // rounds: the pool masks are indexed by round).  The next round of the
+    // row starts right after this one when some lane has candidates there,
+    // else at the lowest candidate left (past a gap between the lanes'
+    // ranges); two ballots settle the common cases (the row is done, or it
+    // goes on at once) without a wave-wide minimum.
+    if (s >= 0) {
+        const int nb = cur + kStage;
+        const bool more = lo_r < hi_r && hi_r > nb;
+        if (__ballot(more) == 0)
+            nxt = kNone;
+        else if (__ballot(more && lo_r < nb + kStage) != 0)
+            nxt = nb;
+        else
+            nxt = wave_min(more ? max(lo_r, nb) : kNone);
+    }
     while (nxt == kNone && s < 8) {
         ++s;
         const int r = row_of(s);
