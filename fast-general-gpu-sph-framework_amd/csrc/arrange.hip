@@ -1097,37 +1097,47 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
 
 // compact[] (CountingSort_Result_two9, sph_arrangement.cu:193-206): TRA
 // particles first in cell order, then SMS particles in cell order; inside a
-// bucket the sorted order.  Block 0 also applies judgeTask's odd-tail rule
+// bucket the sorted order (with no SMS tasks, the identity, written without
+// the per-cell lookups).  Block 0 also applies judgeTask's odd-tail rule
 // and writes the reference's d_middle_value_.
 __global__ __launch_bounds__(kNT) void k_compact(ArrangeArgs a) {
     const int i0 = blockIdx.x * (kNT * kCompactItems) + threadIdx.x;
     const int total_tra = a.dc[kCntTra];
-    // kCompactItems particles per thread, each step's loads issued together
-    int c[kCompactItems], bg[kCompactItems], rq[kCompactItems], sb[kCompactItems],
-        tb[kCompactItems];
+    if (a.dc[kCntSms] == 0) {
+        // no SMS tasks: every particle is TRA, in sorted order
 #pragma unroll
-    for (int u = 0; u < kCompactItems; ++u) {
-        const int i = i0 + u * kNT;
-        c[u] = i < a.n ? (int)(a.hash[i] >> 6) : 0;
-    }
+        for (int u = 0; u < kCompactItems; ++u) {
+            const int i = i0 + u * kNT;
+            if (i < a.n) a.compact[i] = i;
+        }
+    } else {
+        // kCompactItems particles per thread, each step's loads issued together
+        int c[kCompactItems], bg[kCompactItems], rq[kCompactItems], sb[kCompactItems],
+            tb[kCompactItems];
 #pragma unroll
-    for (int u = 0; u < kCompactItems; ++u) {
-        const bool in = i0 + u * kNT < a.n;
-        bg[u] = in ? a.begin[c[u]] : 0;
-        rq[u] = in ? a.req[c[u]] : 0;
-        sb[u] = in ? a.sms_base[c[u]] : 0;
-        tb[u] = in ? a.tra_base[c[u]] : 0;
-    }
+        for (int u = 0; u < kCompactItems; ++u) {
+            const int i = i0 + u * kNT;
+            c[u] = i < a.n ? (int)(a.hash[i] >> 6) : 0;
+        }
 #pragma unroll
-    for (int u = 0; u < kCompactItems; ++u) {
-        const int i = i0 + u * kNT;
-        if (i < a.n) {
-            const int rank = i - bg[u];
-            const int r32 = rq[u] * kSmsTaskParticles;
-            if (rank < r32)
-                a.compact[total_tra + sb[u] + rank] = i;
-            else
-                a.compact[tb[u] + rank - r32] = i;
+        for (int u = 0; u < kCompactItems; ++u) {
+            const bool in = i0 + u * kNT < a.n;
+            bg[u] = in ? a.begin[c[u]] : 0;
+            rq[u] = in ? a.req[c[u]] : 0;
+            sb[u] = in ? a.sms_base[c[u]] : 0;
+            tb[u] = in ? a.tra_base[c[u]] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kCompactItems; ++u) {
+            const int i = i0 + u * kNT;
+            if (i < a.n) {
+                const int rank = i - bg[u];
+                const int r32 = rq[u] * kSmsTaskParticles;
+                if (rank < r32)
+                    a.compact[total_tra + sb[u] + rank] = i;
+                else
+                    a.compact[tb[u] + rank - r32] = i;
+            }
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {

@@ -710,6 +710,16 @@ __device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, i
 // mask word (no pool rounds); it marks every tile it leaves in tile_defer for
 // the general pass.  Without those paths the lean pass fits 80
 // registers (six waves per SIMD instead of five); both give the same bits.
+// The particle of TRA entry e: compact[e], which is e itself when no cell
+// has SMS tasks (ident: every particle on the TRA path, in sorted order;
+// k_compact then writes the identity), so that the load is skipped there.
+// Kernels read the condition once (tra_ident) and pass it to their tiles.
+__device__ __forceinline__ bool tra_ident(const PhysicsArgs &P) { return P.dc[kCntSms] == 0; }
+
+__device__ __forceinline__ int tra_particle(const PhysicsArgs &P, int e, bool ident) {
+    return ident ? e : P.compact[e];
+}
+
 // A tile the lean pass leaves: flagged for the general pass, which takes
 // TRA tiles at all only in an arrange epoch where some tile was left (the
 // epoch is stored once, not by every such tile).
@@ -723,7 +733,7 @@ __device__ __forceinline__ void defer_tile(const PhysicsArgs &P, int it, int lan
 
 template <bool kBatch, int kStage, bool kLean>
 __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int tra, int lane,
-                                             float4 *wl, PairTest pt) {
+                                             float4 *wl, PairTest pt, bool ident) {
     // round walk: row bounds (ints) then the staging round
     int *rlo = reinterpret_cast<int *>(wl), *rhi = rlo + 9 * 64;
     float4 *stg = wl + 2 * 9 * 64 / 4;
@@ -731,7 +741,7 @@ __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int t
         // lanes past the TRA range carry empty rows
         const int idx = it * 64 + lane;
         const bool active = idx < tra;
-        const int i = P.compact[active ? idx : tra - 1];
+        const int i = tra_particle(P, active ? idx : tra - 1, ident);
         const float4 pi = P.pos[i];
         if (kLean) {
             // a lane in a dense cell (four chains) leaves the tile to the
@@ -863,9 +873,10 @@ __global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(kStage > 12
     const int tra = P.dc[kCntTra];
     const ItemMap m = item_map((tra + 63) >> 6);
     const PairTest pt = pair_test(P.c.kernel_2);
+    const bool ident = tra_ident(P);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
-        density_tile<kBatch, kStage, true>(P, it, tra, lane, wl, pt);
+        density_tile<kBatch, kStage, true>(P, it, tra, lane, wl, pt, ident);
     }
 }
 
@@ -884,11 +895,12 @@ __global__ __launch_bounds__(kNT) void k_density(PhysicsArgs P) {
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const PairTest pt = pair_test(P.c.kernel_2);
+    const bool ident = tra_ident(P);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
         if (it < tra_tiles) {
             // the tiles the lean pass left (tile_defer, written by it)
-            if (!P.lean || P.tile_defer[it]) density_tile<kBatch, kStage, false>(P, it, tra, lane, wl, pt);
+            if (!P.lean || P.tile_defer[it]) density_tile<kBatch, kStage, false>(P, it, tra, lane, wl, pt, ident);
         } else {
             const int item = it - tra_tiles;
             const SmsItem sm = sms_item<kBatch>(P, item << 1, T, lane);
@@ -1194,11 +1206,11 @@ __device__ __forceinline__ uint32_t window_bits(uint32_t m, int o, int ws, int w
 
 template <bool kBatch, bool kWin>
 __device__ __forceinline__ void force_tile_lean(const PhysicsArgs &P, int it, int tra, int lane,
-                                                float4 *wl, HH hh) {
+                                                float4 *wl, HH hh, bool ident) {
     const int slot = kWin ? kCntForceAnyDefer2 : kCntForceAnyDefer;
     const int idx = it * 64 + lane;
     const bool active = idx < tra;
-    const int i = P.compact[active ? idx : tra - 1];
+    const int i = tra_particle(P, active ? idx : tra - 1, ident);
     const uint32_t flag = P.use_masks && active ? P.nbmask[(size_t)9 * P.mask_stride + idx] : 0u;
     const bool planned = (flag & kFlagPlan) && !(flag & (kFlagLongRow | kFlagPool));
     if (!P.use_masks || __ballot(active && !planned) != 0) {
@@ -1290,10 +1302,11 @@ void k_force_lean(PhysicsArgs P) {
     const int tiles = kAll || P.dc[kCntForceAnyDefer] == P.dc[kCntEpoch] ? (tra + 63) >> 6 : 0;
     const ItemMap m = item_map(tiles);
     const HH hh = exact_h2(P.c.kernel_h);
+    const bool ident = tra_ident(P);
     for (int it = m.first; it < m.end; it += m.stride) {
         if (!kAll && !P.tile_defer[it]) continue;   // the first pass took it
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
-        force_tile_lean<kBatch, kWin>(P, it, tra, lane, wl, hh);
+        force_tile_lean<kBatch, kWin>(P, it, tra, lane, wl, hh, ident);
     }
 }
 
@@ -1319,6 +1332,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
     const int items = tra_tiles + ((T + 1) >> 1);
     const ItemMap m = item_map(items);
     const HH hh = exact_h2(P.c.kernel_h);
+    const bool ident = tra_ident(P);
     const PairTest pt = pair_test(P.c.kernel_2);
     for (int it = m.first; it < m.end; it += m.stride) {
         wave_sync_lds();   // the previous item's LDS reads precede this item's writes
@@ -1326,7 +1340,7 @@ __global__ __launch_bounds__(kNT) void k_force(PhysicsArgs P) {
         if (it < tra_tiles) {
             const int idx = it * 64 + lane;
             const bool active = idx < tra;
-            const int i = P.compact[active ? idx : tra - 1];
+            const int i = tra_particle(P, active ? idx : tra - 1, ident);
             const float4 pi = P.pos[i];
             const float4 ei = P.ev[i];
             // the density pass's row plan when it stored one, else plan here
