@@ -935,7 +935,12 @@ __global__ __launch_bounds__(kNT) void k_split_count(ArrangeArgs a) {
     block_excl_scan<kNT>(st, s_tmp, tot);
     if (threadIdx.x == 0) a.split_sum[blockIdx.x] = tot;
     block_excl_scan<kNT>(ss, s_tmp, tot);
-    if (threadIdx.x == 0) a.split_sum[a.split_tiles + blockIdx.x] = tot;
+    if (threadIdx.x == 0) {
+        a.split_sum[a.split_tiles + blockIdx.x] = tot;
+        // tag the epoch when this block's cells have SMS work (every such
+        // block stores the same value)
+        if (tot != 0) a.dc[kCntAnySms] = a.dc[kCntEpoch];
+    }
     block_excl_scan<kNT>(sr, s_tmp, tot);
     if (threadIdx.x == 0) a.split_sum[2 * a.split_tiles + blockIdx.x] = tot;
 }
@@ -985,6 +990,16 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
     __shared__ uint32_t s_pre[3];
     const int K = a.dc[kCntK];
     if (blockIdx.x * kSplitTile >= K) return;
+    if (a.dc[kCntAnySms] != a.dc[kCntEpoch]) {
+        // no cell has SMS tasks: every particle is TRA and k_compact writes
+        // the identity, which needs no per-cell bases
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.dc[kCntT] = 0;
+            a.dc[kCntTra] = a.n;
+            a.dc[kCntSms] = 0;
+        }
+        return;
+    }
     const int tid = threadIdx.x;
     uint32_t vs[3];
 #pragma unroll

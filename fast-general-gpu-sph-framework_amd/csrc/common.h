@@ -39,7 +39,8 @@ enum CounterSlot {
     kCntKeysReady = 15, // 1: next_keys holds the micro ids of the slot order's state
     kCntForceAnyDefer = 16,  // the same for the first lean force pass
     kCntForceAnyDefer2 = 17, // and for the second (windowed) one
-    kCntSlots = 18,
+    kCntAnySms = 18,    // the arrange epoch in which some occupied cell last had SMS tasks
+    kCntSlots = 19,
 };
 
 // Geometry of one scene inside the (possibly batched) virtual grid.  In the
