@@ -66,9 +66,9 @@ __device__ __forceinline__ bool micro_coords(float x, float y, float z,
     const float fx = floorf(__fmul_rn(x, rat));
     const float fy = floorf(__fmul_rn(y, rat));
     const float fz = floorf(__fmul_rn(z, rat));
-    const bool ok = fx >= 0.f && fy >= 0.f && fz >= 0.f &&
-                    fx < (float)(g.gx * 4) && fy < (float)(g.gy * 4) &&
-                    fz < (float)(g.gz * 4);
+    // bitwise ands: one mask, no branches
+    const bool ok = (fx >= 0.f) & (fy >= 0.f) & (fz >= 0.f) & (fx < (float)(g.gx * 4)) &
+                    (fy < (float)(g.gy * 4)) & (fz < (float)(g.gz * 4));
     mx = ok ? (int)fx : 0;
     my = ok ? (int)fy : 0;
     mz = ok ? (int)fz : 0;

@@ -464,7 +464,22 @@ __device__ __forceinline__ void density_store(const PhysicsArgs &P, int i, float
         (powf7(rho * c.inv_rest_density) - 1.f) * c.gas_constant;
 }
 
-// The rows of one TRA lane: live flag and sorted range per row.
+// The rows of one TRA lane, a particle whose scene g and micro coordinates
+// are known: the sorted range per row (empty when the row is not live).
+__device__ __forceinline__ void tra_rows_at(const PhysicsArgs &P, const float4 &pi, bool active,
+                                            const SceneGeom &g, int mx, int my, int mz,
+                                            int lo[9], int hi[9]) {
+    const RowSlack sl = row_slack(pi.x, pi.y, pi.z, P.c.cell_size);
+    const bool prune = P.row_prune != 0;
+    row_ranges(g, P.begin, P.xcum, active,
+               [&](int r, int &ny, int &nz, int &gl, int &gr) __attribute__((always_inline)) {
+                   return plan_row(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, prune, sl, r, ny, nz, gl, gr);
+               },
+               lo, hi);
+}
+
+// The same from the particle's index i (scene lookup and micro coordinates
+// here), optionally with its scene's first particle and its cell's count.
 template <bool kBatch>
 __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi, bool active,
                                          int i, int lo[9], int hi[9], int *p_begin = nullptr,
@@ -477,13 +492,7 @@ __device__ __forceinline__ void tra_rows(const PhysicsArgs &P, const float4 &pi,
         const int c = g.cell_base + (mx >> 2) + g.gx * ((my >> 2) + g.gy * (mz >> 2));
         *cell_count = P.begin[c + 1] - P.begin[c];
     }
-    const RowSlack sl = row_slack(pi.x, pi.y, pi.z, P.c.cell_size);
-    const bool prune = P.row_prune != 0;
-    row_ranges(g, P.begin, P.xcum, active,
-               [&](int r, int &ny, int &nz, int &gl, int &gr) __attribute__((always_inline)) {
-                   return plan_row(g, P.c, pi.x, pi.y, pi.z, mx, my, mz, prune, sl, r, ny, nz, gl, gr);
-               },
-               lo, hi);
+    tra_rows_at(P, pi, active, g, mx, my, mz, lo, hi);
 }
 
 // The order in which every path visits a particle's 9 rows (r = 3 (dz + 1)
@@ -743,6 +752,7 @@ __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int t
         const bool active = idx < tra;
         const int i = tra_particle(P, active ? idx : tra - 1, ident);
         const float4 pi = P.pos[i];
+        int lo[9], hi[9], pb = 0, own = 0;
         if (kLean) {
             // a lane in a dense cell (four chains) leaves the tile to the
             // general pass before any row is planned
@@ -754,15 +764,16 @@ __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int t
                 defer_tile(P, it, lane);
                 return;
             }
+            tra_rows_at(P, pi, active, g, mx, my, mz, lo, hi);
+        } else {
+            tra_rows<kBatch>(P, pi, active, i, lo, hi, &pb, &own);
         }
-        int lo[9], hi[9], pb = 0, own = 0;
-        tra_rows<kBatch>(P, pi, active, i, lo, hi, &pb, &own);
         bool lng = false;
 #pragma unroll
         for (int r = 0; r < 9; ++r) lng |= active && hi[r] - lo[r] > kMaskRowMax;
         const bool any_long = __ballot(lng) != 0;
         if (kLean) {
-            if (__ballot(active && own > kFourChainCell) != 0 || (P.use_masks && any_long)) {
+            if (P.use_masks && any_long) {   // (four-chain lanes left above)
                 defer_tile(P, it, lane);
                 return;
             }
