@@ -89,6 +89,7 @@ struct PhysicsArgs {
     int n;
     int nscenes;
     const SceneGeom *scenes;
+    const int *scene_tab;  // [ceil(n / 64)] scene of particle 64 t (batched contexts)
     gsph_constants c;
     float4 *pos;           // sorted state (B)
     f3 *vel;
@@ -168,6 +169,7 @@ void launch_take_flags(int *dc, int *out_host, hipStream_t s);
 void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s);
+void launch_scene_tab(const SceneGeom *scenes, int nscenes, int *tab, int n, hipStream_t s);
 constexpr int kSameCellCap = 512;      // particles of a cell the same-cell pass takes
 void launch_same_cell(const PhysicsArgs &p, hipStream_t s);
 constexpr int kChecksumBlocks = 256;   // blocks of the FP64 checksum's first pass
