@@ -79,6 +79,31 @@ __device__ __forceinline__ int scene_of_cell(const SceneGeom *sc, int ns,
     return lo;
 }
 
+// The same from the layout's tables (scene of every 64th particle / cell),
+// stepping over the scenes that begin in between: two loads instead of a
+// search.  Only batched contexts look anything up.
+template <bool kBatch>
+__device__ __forceinline__ int scene_of_particle_tab(const ArrangeArgs &a, int i) {
+    if (!kBatch) return 0;
+    int s = a.scene_tab[i >> 6];
+    while (s + 1 < a.nscenes && a.scenes[s + 1].p_begin <= i) ++s;
+    return s;
+}
+
+template <bool kBatch>
+__device__ __forceinline__ int scene_of_cell_tab(const ArrangeArgs &a, int c) {
+    if (!kBatch) return 0;
+    int s = a.cell_scene_tab[c >> 6];
+    while (s + 1 < a.nscenes && a.scenes[s + 1].cell_base <= c) ++s;
+    return s;
+}
+
+__global__ void k_scene_tabs(const SceneGeom *sc, int ns, int *tab, int n, int *ctab, int numc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < (n + 63) / 64) tab[t] = scene_of_particle<true>(sc, ns, 64 * t);
+    if (t < (numc + 63) / 64) ctab[t] = scene_of_cell<true>(sc, ns, 64 * t);
+}
+
 // ---------------------------------------------------------------------------
 // The sort, by merging.  Between two steps the state stays in the previous
 // step's sorted order (slot j holds sorted position j), and most particles
@@ -125,10 +150,10 @@ __global__ __launch_bounds__(kNT) void k_hash(ArrangeArgs a) {
     }
     // the wave's 1024 slots usually lie in one scene: look it up once
     int wave_scene = 0;
-    if (kBatch) {
+    if (kBatch && !ready) {
         const int last = min(base + 1023, a.n - 1);
-        const int s0 = scene_of_particle<kBatch>(a.scenes, a.nscenes, min(base, last));
-        wave_scene = s0 == scene_of_particle<kBatch>(a.scenes, a.nscenes, last) ? s0 : -1;
+        const int s0 = scene_of_particle_tab<kBatch>(a, min(base, last));
+        wave_scene = s0 == scene_of_particle_tab<kBatch>(a, last) ? s0 : -1;
     }
     const SceneGeom g_wave = a.scenes[wave_scene >= 0 ? wave_scene : 0];
     uint32_t movers = 0;
@@ -153,7 +178,7 @@ __global__ __launch_bounds__(kNT) void k_hash(ArrangeArgs a) {
                 const uint32_t key =
                     wave_scene >= 0
                         ? key_of(g_wave, pv[s])
-                        : key_of(a.scenes[scene_of_particle<kBatch>(a.scenes, a.nscenes, i)], pv[s]);
+                        : key_of(a.scenes[scene_of_particle_tab<kBatch>(a, i)], pv[s]);
                 mv = !sorted || key != prev[s];
                 if (mv) a.mkey[1][i] = key;
             }
@@ -841,7 +866,7 @@ __global__ __launch_bounds__(kNT) void k_fill(ArrangeArgs a) {
 template <bool kBatch>
 __device__ __forceinline__ int cell_req(const ArrangeArgs &a, int c, int n) {
     if (n == 0) return 0;
-    const SceneGeom g = a.scenes[scene_of_cell<kBatch>(a.scenes, a.nscenes, c)];
+    const SceneGeom g = a.scenes[scene_of_cell_tab<kBatch>(a, c)];
     const int local = c - g.cell_base;
     const int slice = g.gx * g.gy;
     const int nc = slice * g.gz;
@@ -1046,7 +1071,7 @@ __global__ __launch_bounds__(kNT) void k_split_apply(ArrangeArgs a) {
         a.tra_base[c] = (int)tra;
         a.sms_base[c] = (int)sms;
         if (r > 0) {
-            const SceneGeom g = a.scenes[scene_of_cell<kBatch>(a.scenes, a.nscenes, c)];
+            const SceneGeom g = a.scenes[scene_of_cell_tab<kBatch>(a, c)];
             const int local = c - g.cell_base;
             const uint16_t px = (uint16_t)(local % g.gx);
             const uint16_t py = (uint16_t)(local / g.gx % g.gy);
@@ -1274,6 +1299,17 @@ int sample_stride(int n) {
 
 void launch_cells(const ArrangeArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_cells, dim3(a.cell_tiles), dim3(kNT), 0, s, a);
+}
+
+// Scene of every 64th sorted particle and of every 64th cell, for a new
+// layout (the batched lookups scene_of_particle_tab / scene_of_cell_tab and
+// physics' scene_of).
+void launch_scene_tabs(const SceneGeom *scenes, int nscenes, int *tab, int n, int *ctab,
+                       int numc, hipStream_t s) {
+    const int entries = ((n > numc ? n : numc) + 63) / 64;
+    if (entries == 0 || nscenes == 0) return;
+    hipLaunchKernelGGL(k_scene_tabs, dim3((entries + kNT - 1) / kNT), dim3(kNT), 0, s, scenes,
+                       nscenes, tab, n, ctab, numc);
 }
 
 void launch_fill(const ArrangeArgs &a, hipStream_t s) {

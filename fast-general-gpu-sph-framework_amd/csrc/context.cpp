@@ -67,7 +67,8 @@ struct gsph_ctx {
     uint32_t *bcnt = nullptr, *split_sum = nullptr;
     int *cells = nullptr, *cstart = nullptr, *chunk_k0 = nullptr, *fill_val = nullptr;
     uint8_t *tile_defer = nullptr;
-    int *scene_tab = nullptr;   // scene of every 64th sorted particle (batched lookups)
+    int *scene_tab = nullptr;        // scene of every 64th sorted particle (batched lookups)
+    int *cell_scene_tab = nullptr;   // and of every 64th cell
     int *begin = nullptr, *req = nullptr, *tra_base = nullptr, *sms_base = nullptr;
     uint2 *xcum = nullptr;
     int *compact = nullptr;
@@ -215,6 +216,8 @@ ArrangeArgs arrange_args(gsph_ctx *ctx) {
     a.numc = ctx->numc;
     a.nscenes = (int)ctx->scenes.size();
     a.scenes = ctx->scenes_d;
+    a.scene_tab = ctx->scene_tab;
+    a.cell_scene_tab = ctx->cell_scene_tab;
     a.inv_cell_size = ctx->c.inv_cell_size;
     a.split_mode = ctx->opt.split_mode;
     a.sms_min_self = ctx->opt.sms_min_self;
@@ -457,7 +460,8 @@ int configure_layout(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n) {
         HIP_TRY(ctx, hipMemcpyAsync(ctx->scenes_d, ctx->scenes.data(),
                                     sizeof(SceneGeom) * ctx->scenes.size(),
                                     hipMemcpyHostToDevice, ctx->stream));
-        launch_scene_tab(ctx->scenes_d, (int)scenes.size(), ctx->scene_tab, n, ctx->stream);
+        launch_scene_tabs(ctx->scenes_d, (int)scenes.size(), ctx->scene_tab, n, ctx->cell_scene_tab,
+                          ctx->numc, ctx->stream);
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     }
     // launch sizes changed: a captured step graph is stale
@@ -508,6 +512,7 @@ size_t buffer_bytes(const gsph_options &o, size_t N, size_t C, size_t S) {
     t += b(cell_tiles, 4) + b(3 * split_tiles, 4);                    // bcnt split_sum
     t += 2 * b(N + 1, 4) + 2 * b(nchunks + 1, 4);                     // cells cstart chunk_k0 fill_val
     t += b(C + 1, 4) + 3 * b(C, 4) + b(C, 8);                         // begin req tra sms xcum
+    t += b((C + 63) / 64, 4);                                         // cell_scene_tab
     t += b(N, 4) + b((size_t)kMaskWords * N, 4);                      // compact nbmask
     t += b((N + 63) / 64, 1) + b((N + 63) / 64, 4);                   // tile_defer scene_tab
     t += b(rows * 64, 4) + b(task_cap / 2 + 2, 4);                    // sms_pool sms_off
@@ -570,6 +575,7 @@ int alloc_buffers(gsph_ctx *ctx, int cap_n, int cap_c, int cap_s) {
     ALLOC(ctx->compact, N);
     ALLOC(ctx->tile_defer, (N + 63) / 64);
     ALLOC(ctx->scene_tab, (N + 63) / 64);
+    ALLOC(ctx->cell_scene_tab, (C + 63) / 64);
     ALLOC(ctx->nbmask, (size_t)kMaskWords * N);
     // SMS masks: about one 32-candidate sub-batch row per 8 bytes of budget
     // per particle (rows hold 64 lanes x 4 B)
@@ -812,7 +818,9 @@ int configure_layout_async(gsph_ctx *ctx, std::vector<SceneGeom> scenes, int n,
         if (sb) HIP_TRY(ctx, hipMemcpyAsync(ctx->segs, host + gb, sb, hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipEventRecord(done, ctx->stream));
     }
-    if (gb) launch_scene_tab(ctx->scenes_d, (int)scenes.size(), ctx->scene_tab, n, ctx->stream);
+    if (gb)
+        launch_scene_tabs(ctx->scenes_d, (int)scenes.size(), ctx->scene_tab, n, ctx->cell_scene_tab,
+                          (int)numc, ctx->stream);
     ctx->scenes = scenes;
     ctx->n = n;
     ctx->numc = (int)numc;

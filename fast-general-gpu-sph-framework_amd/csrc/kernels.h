@@ -35,6 +35,8 @@ struct ArrangeArgs {
     int numc;              // cells of the (virtual) grid
     int nscenes;
     const SceneGeom *scenes;
+    const int *scene_tab;       // [ceil(n / 64)] scene of particle 64 t (batched contexts)
+    const int *cell_scene_tab;  // [ceil(numc / 64)] scene of cell 64 t
     float inv_cell_size;
     int split_mode;        // 0 reference heuristic, 1 tuned thresholds
     int sms_min_self, sms_min_total;
@@ -169,7 +171,8 @@ void launch_take_flags(int *dc, int *out_host, hipStream_t s);
 void launch_density(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_force(const PhysicsArgs &p, bool batch, hipStream_t s);
 void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s);
-void launch_scene_tab(const SceneGeom *scenes, int nscenes, int *tab, int n, hipStream_t s);
+void launch_scene_tabs(const SceneGeom *scenes, int nscenes, int *tab, int n, int *ctab,
+                       int numc, hipStream_t s);
 constexpr int kSameCellCap = 512;      // particles of a cell the same-cell pass takes
 void launch_same_cell(const PhysicsArgs &p, hipStream_t s);
 constexpr int kChecksumBlocks = 256;   // blocks of the FP64 checksum's first pass

@@ -56,22 +56,18 @@ __device__ __forceinline__ int scene_of_particle(const SceneGeom *sc, int ns,
     return lo;
 }
 
-// The scene of sorted particle i in a batched context: the table gives the
-// scene of particle 64 (i / 64), and the scenes that begin between there and
-// i (none, but where a scene starts inside those 64) are stepped over.  The
-// same answer as scene_of_particle's search, in two loads instead of one per
-// halving of the scene count.
+// The scene of sorted particle i in a batched context: the table (built by
+// launch_scene_tabs with each layout) gives the scene of particle
+// 64 (i / 64), and the scenes that begin between there and i (none, but where
+// a scene starts inside those 64) are stepped over.  The same answer as
+// scene_of_particle's search, in two loads instead of one per halving of the
+// scene count.
 template <bool kBatch>
 __device__ __forceinline__ int scene_of(const PhysicsArgs &P, int i) {
     if (!kBatch) return 0;
     int s = P.scene_tab[i >> 6];
     while (s + 1 < P.nscenes && P.scenes[s + 1].p_begin <= i) ++s;
     return s;
-}
-
-__global__ void k_scene_tab(const SceneGeom *sc, int ns, int *tab, int n) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < (n + 63) / 64) tab[t] = scene_of_particle<true>(sc, ns, 64 * t);
 }
 
 // #particles of a cell (begin b0, next cell's begin b1, x-layer starts x)
@@ -1895,14 +1891,6 @@ void launch_integrate(const PhysicsArgs &p, bool batch, hipStream_t s) {
         hipLaunchKernelGGL(k_integrate<true>, dim3(blocks), dim3(kNT), 0, s, p);
     else
         hipLaunchKernelGGL(k_integrate<false>, dim3(blocks), dim3(kNT), 0, s, p);
-}
-
-// Scene of every 64th sorted particle, for a new layout (scene_of).
-void launch_scene_tab(const SceneGeom *scenes, int nscenes, int *tab, int n, hipStream_t s) {
-    const int entries = (n + 63) / 64;
-    if (entries == 0 || nscenes == 0) return;
-    hipLaunchKernelGGL(k_scene_tab, dim3((entries + kNT - 1) / kNT), dim3(kNT), 0, s, scenes,
-                       nscenes, tab, n);
 }
 
 // ---------------------------------------------------------------------------
