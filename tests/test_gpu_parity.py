@@ -833,6 +833,39 @@ def test_batch_alone_vs_mix(gsph, po):
     b.close()
 
 
+def test_batch_tiny_scenes_alone_vs_mix(gsph, po):
+    """Scenes smaller than one 64-particle tile (several scenes start inside
+    one tile, and inside one 64-particle block of the layout's scene table)
+    give bitwise the results they give alone."""
+    op = make_params(h=0.02, world=(1, 1, 1))
+    dims = [(2, 2, 2), (1, 1, 3), (3, 3, 3), (1, 1, 1), (4, 4, 4), (2, 3, 4), (5, 5, 5), (1, 2, 1)]
+    worlds, chunks = [], []
+    for k, (nx, ny, nz) in enumerate(dims):
+        pos = po.scene_block([0.06, 0.06, 0.06], nx, ny, nz, 0.01, 0.01, 300 + k)
+        ext = pos[:, :3].max(0) - 0.06 + 0.01
+        worlds.append([max(0.2, 2 * e + 0.12) for e in ext])
+        chunks.append(pos)
+    counts = [len(c) for c in chunks]
+    gp = to_gsph_params(gsph, op)
+    banner(f"batch tiny scenes counts={counts}")
+    b = gsph.System(gp, np.concatenate(chunks), batch_worlds=worlds, batch_counts=counts)
+    b.step(3)
+    bp, bv, be, ba = b.read_state()
+    for k, c in enumerate(chunks):
+        gp1 = to_gsph_params(gsph, op)
+        for a in range(3):
+            gp1.world_size[a] = worlds[k][a]
+        s = gsph.System(gp1, c)
+        s.step(3)
+        sp, sv, se, sa = s.read_state()
+        f, cnt = b.scene_range(k)
+        assert cnt == len(c)
+        for x, y in ((bp, sp), (bv, sv), (be, se), (ba, sa)):
+            assert np.array_equal(x[f:f + cnt], y)
+        s.close()
+    b.close()
+
+
 def test_config1_full_size_properties(gsph, po):
     """Workload 1 at full size (3,944,312 particles, 317^3 cells): the sort
     against the oracle's keys, the first step against the FP64 oracle, the
