@@ -727,12 +727,6 @@ __device__ __forceinline__ void density_row(const float4 &pi, const float4 *q, i
     }
 }
 
-// One TRA tile of the density pass: 64 consecutive entries of compact[0,
-// tra), one particle per lane.  kLean (the lean pass) takes a tile only when
-// no lane sums in four chains and, with pair masks on, no row exceeds one
-// mask word (no pool rounds); it marks every tile it leaves in tile_defer for
-// the general pass.  Without those paths the lean pass fits 80
-// registers (six waves per SIMD instead of five); both give the same bits.
 // The particle of TRA entry e: compact[e], which is e itself when no cell
 // has SMS tasks (ident: every particle on the TRA path, in sorted order;
 // k_compact then writes the identity), so that the load is skipped there.
@@ -754,6 +748,12 @@ __device__ __forceinline__ void defer_tile(const PhysicsArgs &P, int it, int lan
     }
 }
 
+// One TRA tile of the density pass: 64 consecutive entries of compact[0,
+// tra), one particle per lane.  kLean (the lean pass) takes a tile only when
+// no lane sums in four chains and, with pair masks on, no row exceeds one
+// mask word (no pool rounds); it marks every tile it leaves in tile_defer for
+// the general pass.  Without those paths the lean pass fits 80
+// registers (six waves per SIMD instead of five); both give the same bits.
 template <bool kBatch, int kStage, bool kLean>
 __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int tra, int lane,
                                              float4 *wl, PairTest pt, bool ident) {
