@@ -1837,7 +1837,12 @@ void launch_force_t(const PhysicsArgs &p, hipStream_t s) {
         hipLaunchKernelGGL((k_force_lean<B, true, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
     } else if (p.lean) {
         hipLaunchKernelGGL((k_force_lean<B, false, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
-        hipLaunchKernelGGL((k_force_lean<B, true, false>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
+        // the windowed pass takes only the planned tiles the first left (few
+        // or none, never SMS items): a quarter of the grid, still more blocks
+        // than every CU holds at once, costs a quarter of the block dispatch
+        // (the item schedule serves any multiple of 8 blocks)
+        const int gb = std::max(8, (p.grid_blocks / 4) & ~7);
+        hipLaunchKernelGGL((k_force_lean<B, true, false>), dim3(gb), dim3(kNT), 0, s, p);
     }
     if (p.sms_legacy)
         hipLaunchKernelGGL((k_force<B, S, true>), dim3(p.grid_blocks), dim3(kNT), 0, s, p);
