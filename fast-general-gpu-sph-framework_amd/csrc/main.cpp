@@ -43,7 +43,7 @@ int main(int argc, char **argv) {
     int frames = 200;  // headless default (main.cpp:44)
     sph::SceneSource src;
     bool scene_given = false, autotune = false;
-    int sms_legacy = 0, local_merge = 1, same_cell = 0;
+    int sms_legacy = 0, local_merge = 1, same_cell = 0, lean = 1;
     for (int i = 1; i < argc; ++i) {
         const bool has_value = i + 1 < argc;
         if (!std::strcmp(argv[i], "--autotune")) {
@@ -54,6 +54,8 @@ int main(int argc, char **argv) {
             local_merge = 0;
         } else if (!std::strcmp(argv[i], "--same-cell-pair-force")) {
             same_cell = 1;
+        } else if (!std::strcmp(argv[i], "--no-lean-passes")) {
+            lean = 0;
         } else if (has_value &&
                    (!std::strcmp(argv[i], "--headless") || !std::strcmp(argv[i], "--benchmark"))) {
             frames = std::atoi(argv[++i]);
@@ -85,6 +87,7 @@ int main(int argc, char **argv) {
         o.sms_legacy_staging = sms_legacy;
         o.sms_local_merge = local_merge;
         o.same_cell_pair_force = same_cell;
+        o.lean_passes = lean;
         if (gsph_set_options(system.context(), &o) != GSPH_OK) {
             std::fprintf(stderr, "%s\n", gsph_last_error(system.context()));
             return EXIT_FAILURE;
@@ -97,10 +100,10 @@ int main(int argc, char **argv) {
             std::fprintf(stderr, "%s\n", gsph_last_error(system.context()));
             return EXIT_FAILURE;
         }
-        std::printf("Autotuned: %d blocks per CU, staging depth %d, pair masks %d, split %s "
-                    "(%.3f ms per trial step)\n",
+        std::printf("Autotuned: %d blocks per CU, staging depth %d, pair masks %d, split %s, "
+                    "lean passes %d (%.3f ms per trial step)\n",
                     chosen.blocks_per_cu, chosen.stage_depth, chosen.pair_masks,
-                    chosen.split_mode == 0 ? "reference" : "tuned", ms);
+                    chosen.split_mode == 0 ? "reference" : "tuned", chosen.lean_passes, ms);
     }
     system.runBenchmark(frames);
     return EXIT_SUCCESS;

@@ -39,6 +39,13 @@ def test_cli_report_and_checksum(gsph, tmp_path):
     assert re.search(r"Hybrid split: 32768 TRA particles, 0 SMS tasks", out)
 
 
+def test_cli_no_lean_passes_same_checksum(tmp_path):
+    """--no-lean-passes sends every tile to the general kernels: same state."""
+    _, cs = run(["--config", "0", "--benchmark", "12"], tmp_path)
+    _, cs_g = run(["--config", "0", "--benchmark", "12", "--no-lean-passes"], tmp_path)
+    assert np.array_equal(cs, cs_g), (cs, cs_g)
+
+
 def test_cli_default_scene_and_autotune(tmp_path):
     """No --scene, run outside the repository: the reference's own scene
     shipped with the package; --autotune changes the launch options and the
