@@ -118,6 +118,32 @@ def test_arrange_bit_exact(gsph, po, name):
     s.close()
 
 
+def test_split_switches_between_arranges(gsph, po):
+    """SMS work appears, disappears and appears again in one context (the
+    split rule switched between arranges): each arrange's grid is the
+    oracle's, and without SMS tasks the compact index is the identity, every
+    req zero, no task and no middle."""
+    op, pos = scene(po, "mixed")
+    n = len(pos)
+    ref = po.arrange(op, pos)
+    assert ref["tasks_count"] > 0
+    tra_only = dict(ref)
+    tra_only.update(compact=np.arange(n, dtype=ref["compact"].dtype), req=np.zeros_like(ref["req"]),
+                    middle_raw=-1, tasks_count=0, tasks=ref["tasks"][:0])
+    banner(f"split switches n={n}")
+    s = make_system(gsph, op, pos)
+    for k, opts in enumerate(({}, ALL_TRA, {}, ALL_TRA, {})):
+        o = gsph.options_default()
+        for key, v in opts.items():
+            setattr(o, key, v)
+        s.set_options(o)
+        if k:
+            s.set_state(pos)   # the same input again (arrange follows integrate or a new state)
+        s.arrange()
+        compare_grid(s.read_grid(), tra_only if opts else ref)
+    s.close()
+
+
 def test_arrange_reference_default_scene(gsph, po):
     """The reference's own scene (3,944,312 particles, ~296 per cell, SMS)."""
     from pathlib import Path
