@@ -769,16 +769,19 @@ __device__ __forceinline__ void density_tile(const PhysicsArgs &P, int it, int t
         int lo[9], hi[9], pb = 0, own = 0;
         if (kLean) {
             // a lane in a dense cell (four chains) leaves the tile to the
-            // general pass before any row is planned
+            // general pass; its cell's count is loaded ahead of the row
+            // lookups and tested after them, so the rows wait for no extra
+            // round trip
             int mx, my, mz;
             const SceneGeom g = P.scenes[scene_of<kBatch>(P, i)];
             micro_coords(pi.x, pi.y, pi.z, P.c.inv_cell_size, g, mx, my, mz);
             const int c = g.cell_base + (mx >> 2) + g.gx * ((my >> 2) + g.gy * (mz >> 2));
-            if (__ballot(active && P.begin[c + 1] - P.begin[c] > kFourChainCell) != 0) {
+            const int own0 = P.begin[c], own1 = P.begin[c + 1];
+            tra_rows_at(P, pi, active, g, mx, my, mz, lo, hi);
+            if (__ballot(active && own1 - own0 > kFourChainCell) != 0) {
                 defer_tile(P, it, lane);
                 return;
             }
-            tra_rows_at(P, pi, active, g, mx, my, mz, lo, hi);
         } else {
             tra_rows<kBatch>(P, pi, active, i, lo, hi, &pb, &own);
         }
